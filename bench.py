@@ -1,0 +1,173 @@
+"""Throughput of the PowerLLEL channel-flow RK2 step on one MI355X.
+
+Workload: the Re_tau~550 channel, 768 x 384 x 384 (2pi x pi x 2, tanh stretch
+1.8, Re_b = 20000 -> re = 10000, dt = 5e-4), seeded log profile + 10% noise,
+synthetic data.  One step = the full sequence of src/main.cc:116-154 (RK1,
+halo/BC, RK2, halo/BC, Poisson RHS, FFT/tridiagonal Poisson solve, pressure
+BC, projection, constant-flow forcing, halo/BC).
+
+Metric: Mpoint-steps/s = nx*ny*nz*steps / seconds / 1e6, inputs resident in
+HBM.  The roofline entry reports the dominant kernel's algorithmic bytes per
+launch over its average HIP-event duration against the 8 TB/s HBM peak.  The
+CPU baseline is the FP64 oracle (the CPU restatement of the reference
+algorithm) timed for one step of the same grid on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# name: (nx, ny, nz, lx, ly, lz, stretch, re, dt, profile)
+WORKLOADS = {
+    "retau550_768x384x384": (768, 384, 384, 6.283185307179586, 3.141592653589793, 2.0, 1.8,
+                             10000.0, 5e-4, "log"),
+    "retau180_256x192x128": (256, 192, 128, 12.566370614359172, 6.283185307179586, 2.0, 1.5,
+                             2800.0, 1e-3, "poi"),
+    "retau180_64x64x64": (64, 64, 64, 6.283185307179586, 3.141592653589793, 2.0, 1.5, 2800.0,
+                          2e-3, "poi"),
+}
+DEFAULT_WORKLOAD = "retau550_768x384x384"
+
+# Algorithmic (compulsory) HBM bytes per grid point per launch, FP64; FP32 halves them.
+# RK1 reads u,v,w and writes u1,v1,w1; RK2 reads u1,v1,w1,u,v,w and writes u,v,w;
+# calcRHS reads u,v,w and writes p; each transform and the tridiagonal solve read
+# and write the work array once; correctVel reads p,u,v,w and writes u,v,w; the
+# bulk mean reads u; the force update reads and writes u.
+BYTES_PER_POINT_FP64 = {
+    "rk1": 48, "rk2": 72, "calc_rhs": 32, "fft_x_fwd": 16, "fft_y_fwd": 16, "solve_trid": 16,
+    "fft_y_bwd": 16, "fft_x_bwd": 16, "correct_vel": 56, "mean_partial": 8, "force_add": 16,
+}
+STEP_BYTES_FP64 = 312
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    return ap.parse_args()
+
+
+def build_channel(wl, precision):
+    import powerllel_channel_sycl_amd as pll
+    nx, ny, nz, lx, ly, lz, r, re, dt, prof = WORKLOADS[wl]
+    dzf = pll.mesh_by_func(1, 0, r, nz, lz)
+    ch = pll.Channel(nx, ny, nz, lx, ly, lz, re, dt, dzf, precision=precision)
+    ch.initFlow(prof, noise_intensity=0.1, seeds=(42, 43, 44))
+    ch.updateBoundVel(0)
+    ch.updateBoundP()
+    return ch, nx * ny * nz
+
+
+def cpu_baseline(wl, steps):
+    """FP64 oracle on this host: `steps` RK2 steps of the full workload grid."""
+    from oracle import pll_oracle as po
+    nx, ny, nz, lx, ly, lz, r, re, dt, prof = WORKLOADS[wl]
+    dzf = po.mesh_by_func(1, 0, r, nz, lz)
+    o = po.Oracle(nx, ny, nz, lx, ly, lz, re, dt, dzf)
+    u, v, w, u1, v1, w1, p = (o.field() for _ in range(7))
+    o.init_flow(u, v, w, prof, noise_intensity=0.1)
+    o.update_bound_vel(u, v, w)
+    o.update_bound_p(p)
+    t0 = time.perf_counter()
+    o.step(steps, u, v, w, u1, v1, w1, p)
+    sec = time.perf_counter() - t0
+    return {
+        "value": nx * ny * nz * steps / sec / 1e6,
+        "unit": "Mpoint-steps/s",
+        "cores": int(po.lib().orc_num_threads()),
+        "kind": "port",
+        "sample": f"{steps} FP64 RK2 step(s) of the full {nx}x{ny}x{nz} grid, "
+                  f"{sec:.2f} s, OpenMP oracle",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is a design, see DESIGN.md)"}))
+        sys.exit(2)
+    ch, npts = build_channel(args.workload, args.precision)
+    print(f"# workload {args.workload} precision {args.precision}: {npts} points, "
+          f"{ch.device_bytes() / 1e9:.2f} GB on device", flush=True)
+    ch.step(args.warmup)
+    ch.synchronize()
+    t0 = time.perf_counter()
+    ch.step(args.steps)
+    ch.synchronize()
+    sec = time.perf_counter() - t0
+    ms_step = 1e3 * sec / args.steps
+    value = npts * args.steps / sec / 1e6
+    print(f"# timed {args.steps} steps: {ms_step:.3f} ms/step", flush=True)
+
+    # per-kernel HIP-event timings over a separate window of the same steps
+    ch.timing(True)
+    ch.timing_reset()
+    ch.step(args.steps)
+    ch.synchronize()
+    times = ch.timing_read()
+    ch.timing(False)
+    scale = 1.0 if args.precision == "fp64" else 0.5
+    kernels = {}
+    for name, (ms, n) in times.items():
+        if n == 0:
+            continue
+        avg_ms = ms / n
+        bpp = BYTES_PER_POINT_FP64.get(name)
+        entry = {"avg_us": round(avg_ms * 1e3, 2), "launches": n, "share": None}
+        if bpp is not None:
+            gbs = bpp * scale * npts / (avg_ms * 1e-3) / 1e9
+            entry["GBps"] = round(gbs, 1)
+            entry["frac"] = round(gbs / HBM_PEAK_GBS, 4)
+        kernels[name] = entry
+    total = sum(ms for ms, n in times.values() if n)
+    for name, (ms, n) in times.items():
+        if n:
+            kernels[name]["share"] = round(ms / total, 4)
+    dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: times[k][0])
+    d_avg_s = times[dom][0] / times[dom][1] * 1e-3
+    d_bytes = BYTES_PER_POINT_FP64[dom] * scale * npts
+    achieved = d_bytes / d_avg_s / 1e9
+    step_frac = STEP_BYTES_FP64 * scale * npts / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+
+    result = {
+        "metric": "Mpoint-steps/s per RK2 step (FP64 vs FP32-mixed) + % HBM roofline, 1 GPU",
+        "value": round(value, 2),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded log profile + 10% uniform noise)",
+        "config": {"workload": args.workload, "grid": list(WORKLOADS[args.workload][:3]),
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "step_frac": round(step_frac, 4),
+                     "bytes_per_point_step": STEP_BYTES_FP64 * scale},
+        "kernels": kernels,
+    }
+    ch.close()
+    if not args.no_cpu_baseline:
+        print("# cpu baseline ...", flush=True)
+        result["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

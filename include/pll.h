@@ -1,0 +1,198 @@
+/*
+ * pll.h -- C-ABI of the MI355X-native PowerLLEL channel-flow hot path.
+ *
+ * One opaque context owns every device buffer of one channel-flow problem on
+ * one GPU: the seven halo-padded fields (u, v, w, u1, v1, w1, p), the mesh
+ * coefficients, the FFT twiddles, the pre-factored tridiagonal pivots and the
+ * reduction scratch.  Each entry point replaces one routine of the reference's
+ * module interface; the citation on each line is the routine it replaces
+ * (paths relative to the PowerLLEL_Channel_SYCL source tree).
+ *
+ * Layout contract (src/mod_type.hh:107-117,172-179): a field is
+ * (nx+2)*(ny+2)*(nz+2) values, x fastest, one halo cell per side, logical
+ * indices 0..n+1.  Host buffers passed to the upload/download calls use this
+ * layout in FP64, whatever the context's precision.  The Poisson work array
+ * (var_xpen, src/mod_poisson_solver.cc:196-214) is nx*ny*nz values ordered
+ * [k][j][i] with the reference's "perm" packing of each real DFT
+ * (src/mod_fft.cc:79-86).
+ *
+ * Semantics: calls are ordered on the context's HIP stream and return
+ * before the GPU finishes, except those that hand a value back to the host
+ * (means, checks, downloads), which synchronise.  pll_synchronize() waits for
+ * everything queued.  Every call returns PLL_OK or a pll_status code;
+ * pll_last_error() describes the last failure on the calling thread.  Nothing
+ * aborts the process.  One context must not be driven by two host threads at
+ * the same time.
+ */
+#ifndef PLL_H
+#define PLL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLL_ABI_VERSION 1
+
+typedef struct pll_context pll_context;
+
+typedef enum pll_status {
+  PLL_OK = 0,
+  PLL_ERR_ARG = 1,         /* invalid argument or unsupported size          */
+  PLL_ERR_HIP = 2,         /* a HIP runtime call failed                     */
+  PLL_ERR_STATE = 3,       /* call not valid in the context's current state */
+  PLL_ERR_UNSUPPORTED = 4, /* feature not available in this build           */
+  PLL_ERR_NOMEM = 5        /* device allocation failed                      */
+} pll_status;
+
+typedef enum pll_precision {
+  PLL_FP64 = 0,      /* every field, transform and solve in FP64           */
+  PLL_FP32_MIXED = 1 /* FP32 fields/stencils/FFTs; FP64 for the kx=ky=0 and
+                        low-wavenumber tridiagonal solves and for the
+                        bulk-velocity reduction                            */
+} pll_precision;
+
+typedef enum pll_field {
+  PLL_U = 0, PLL_V = 1, PLL_W = 2,
+  PLL_U1 = 3, PLL_V1 = 4, PLL_W1 = 5,
+  PLL_P = 6,
+  PLL_NFIELD = 7
+} pll_field;
+
+typedef enum pll_profile { PLL_PROFILE_UNI = 0, PLL_PROFILE_POI = 1, PLL_PROFILE_LOG = 2 } pll_profile;
+
+/* Problem definition: the PARA/MESH/TIME/PHYSICAL/FORCE keys of the
+ * reference's param.json that the hot path reads (src/mod_parameters.hh:12-88). */
+typedef struct pll_config {
+  int nx, ny, nz;          /* global grid; even (src/mod_parameters.cc:170-176)   */
+  double lx, ly, lz;
+  double re, u_ref, l_ref; /* re_inv = u_ref*l_ref/re (src/mod_parameters.cc:259) */
+  double dt;
+  double u_crf;            /* convecting reference frame velocity                */
+  int is_forced[3];
+  double vel_force[3];
+  int precision;           /* pll_precision                                      */
+  int pin_singular;        /* 1: the kx=ky=0 Neumann pivot is exactly zero, so the
+                              solve takes the reference's b==0 branch
+                              (src/mod_poisson_solver.cc:341-346)              */
+  int device;              /* HIP device ordinal                                 */
+} pll_config;
+
+/* Max-with-location record of the runtime checks (src/mod_utils.hh:8-12). */
+typedef struct pll_value_index {
+  double value;
+  int rank;
+  int ig, jg, kg;
+} pll_value_index;
+
+/* ---- library --------------------------------------------------------------- */
+int pll_abi_version(void);
+const char *pll_last_error(void);
+/* Fills cfg with the defaults of examples/template/param.json. */
+void pll_config_default(pll_config *cfg);
+
+/* ---- mod_mesh ---------------------------------------------------------------- */
+/* initMeshByFunc (src/mod_mesh.cc:95-125): dzf_global[0..nz+1]. */
+int pll_mesh_by_func(int mesh_type, int stretch_func, double stretch_ratio, int nz, double lz,
+                     double *dzf_global);
+
+/* ---- context (allocVariables + initMesh + init_poisson_solver) -------------- */
+/* allocVariables (src/mod_variables.cc:4-12), initMesh coefficients
+ * (src/mod_mesh.cc:54-65) and init_poisson_solver
+ * (src/mod_poisson_solver.cc:180-315) in one call.  dzf_global is the
+ * halo-inclusive spacing array dzf_global[0..nz+1] (src/main.cc:55-57). */
+int pll_create(const pll_config *cfg, const double *dzf_global, pll_context **out);
+/* free_poisson_solver + freeVariables + freeMesh (src/mod_poisson_solver.cc:512-537). */
+int pll_destroy(pll_context *ctx);
+int pll_get_config(const pll_context *ctx, pll_config *cfg);
+/* Bytes of device memory the context holds. */
+int pll_device_bytes(const pll_context *ctx, size_t *bytes);
+
+/* ---- field access (Array3D::data(), src/mod_type.hh:169-170) ----------------- */
+int pll_field_upload(pll_context *ctx, int field, const double *host);
+int pll_field_download(pll_context *ctx, int field, double *host);
+/* Device address of a field and its element size (8 for FP64, 4 for FP32). */
+int pll_field_device_ptr(pll_context *ctx, int field, void **dev, int *elem_bytes);
+/* Poisson work array var_xpen (nx*ny*nz, [k][j][i]). */
+int pll_spec_upload(pll_context *ctx, const double *host);
+int pll_spec_download(pll_context *ctx, double *host);
+
+/* ---- mod_initFlow ------------------------------------------------------------ */
+/* initFlow (src/mod_initFlow.cc:47-102) with this library's documented noise
+ * stream: for interior point L = (k-1)*nx*ny + (j-1)*nx + (i-1) of component
+ * d, a std::minstd_rand engine seeded with seeds[d] is advanced L steps and
+ * drawn once (x); the point gets 2*((x-1)/2147483646 - 0.5)*noise_intensity.
+ * Keeps the reference's Mean(nz, ny, ...) normalisation (src/mod_initFlow.cc:92). */
+int pll_init_flow(pll_context *ctx, int profile, double u0, int with_noise,
+                  double noise_intensity, const uint32_t seeds[3]);
+
+/* ---- mod_calcVel -------------------------------------------------------------- */
+/* transform2CRF (src/mod_calcVel.cc:13-25) on u; vel_force[0] -= vel_crf. */
+int pll_transform2crf(pll_context *ctx, double vel_crf);
+/* timeIntVelRK1 (src/mod_calcVel.cc:131-138): u1,v1,w1 = u + dt*F(u). */
+int pll_time_int_vel_rk1(pll_context *ctx);
+/* timeIntVelRK2 (src/mod_calcVel.cc:244-251): u = (u1 + dt*F(u1) + u)/2. */
+int pll_time_int_vel_rk2(pll_context *ctx);
+/* correctVel (src/mod_calcVel.cc:253-278): u -= dt*grad(p). */
+int pll_correct_vel(pll_context *ctx);
+/* forceVel (src/mod_calcVel.cc:280-320).  mean_out (may be NULL) receives the
+ * bulk means of the forced components; passing NULL keeps the call asynchronous. */
+int pll_force_vel(pll_context *ctx, double mean_out[3]);
+
+/* ---- mod_calcRHS -------------------------------------------------------------- */
+/* calcRHS (src/mod_calcRHS.cc:7-33): p = div(u)/dt. */
+int pll_calc_rhs(pll_context *ctx);
+
+/* ---- poisson_solver / mod_fft ------------------------------------------------- */
+/* execute_poisson_solver (src/mod_poisson_solver.cc:454-510) on p. */
+int pll_poisson_execute(pll_context *ctx);
+/* execute_fft_fwd_xpen (src/mod_fft.cc:60-88): p interior -> work array. */
+int pll_fft_fwd_xpen(pll_context *ctx);
+/* execute_fft_fwd_ypen (src/mod_fft.cc:122-153): work array, in place. */
+int pll_fft_fwd_ypen(pll_context *ctx);
+/* ssolve_trid (src/mod_poisson_solver.cc:317-357): work array, in place. */
+int pll_solve_trid(pll_context *ctx);
+/* execute_fft_bwd_ypen (src/mod_fft.cc:155-187): work array, in place. */
+int pll_fft_bwd_ypen(pll_context *ctx);
+/* execute_fft_bwd_xpen (src/mod_fft.cc:90-120): work array -> p interior, scaled 1/(nx*ny). */
+int pll_fft_bwd_xpen(pll_context *ctx);
+/* get_eigen_values (src/mod_fft.cc:196-222): perm-ordered eigenvalues, unscaled. */
+int pll_eigen_values(int n, double *lambda);
+
+/* ---- mod_updateBound ---------------------------------------------------------- */
+/* updateBoundVel (src/mod_updateBound.cc:229-251): which = 0 -> (u,v,w), 1 -> (u1,v1,w1). */
+int pll_update_bound_vel(pll_context *ctx, int which);
+/* updateBoundP (src/mod_updateBound.cc:253-296). */
+int pll_update_bound_p(pll_context *ctx);
+
+/* ---- mod_utils ------------------------------------------------------------------ */
+/* Mean (src/mod_utils.cc:30-51): sum(var*dzflzi)/nx_global/ny_global, FP64. */
+int pll_mean(pll_context *ctx, int field, int nx_global, int ny_global, double *out);
+/* calcMaxDiv (src/mod_utils.cc:164-205) and calcMaxCFL (src/mod_utils.cc:108-150). */
+int pll_calc_max_div(pll_context *ctx, pll_value_index *out);
+int pll_calc_max_cfl(pll_context *ctx, pll_value_index *out);
+/* checkNaN (src/mod_utils.cc:77-106): *passed = 1 when no NaN in the whole array. */
+int pll_check_nan(pll_context *ctx, int field, int *passed);
+
+/* ---- main loop ------------------------------------------------------------------ */
+/* nsteps repetitions of src/main.cc:116-154: RK1, BCvel(u1), RK2, BCvel(u),
+ * calcRHS, Poisson, BCp, correctVel, forceVel, BCvel(u).  Asynchronous. */
+int pll_step(pll_context *ctx, int nsteps);
+int pll_synchronize(pll_context *ctx);
+
+/* ---- per-kernel timing (HIP events on the context's stream) ------------------- */
+/* When enabled, each kernel launch is bracketed by HIP events; totals are
+ * accumulated at pll_timing_read().  Names are stable identifiers. */
+int pll_timing_enable(pll_context *ctx, int on);
+int pll_timing_reset(pll_context *ctx);
+int pll_timing_count(pll_context *ctx, int *nkernels);
+int pll_timing_read(pll_context *ctx, int index, const char **name, double *total_ms,
+                    long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLL_H */

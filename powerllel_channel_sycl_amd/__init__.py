@@ -1,0 +1,323 @@
+"""MI355X-native PowerLLEL channel-flow hot path.
+
+The compute lives in ``libpll.so`` (hand-written HIP kernels for gfx950 behind
+the C-ABI of ``include/pll.h``).  This module is the Python host side over that
+C-ABI: a :class:`Channel` owns one device context and exposes one method per
+routine of the reference's module interface, under the reference's names
+(``timeIntVelRK1``, ``calcRHS``, ``execute_poisson_solver``, ``updateBoundVel``
+...; src/mod_calcVel.hh, src/mod_calcRHS.hh, src/mod_poisson_solver.hh,
+src/mod_updateBound.hh, src/mod_utils.hh).
+
+Fields cross the boundary as float64 numpy arrays of shape (nz+2, ny+2, nx+2)
+in the reference layout (src/mod_type.hh:107-117).  Failures raise
+:class:`PllError` carrying the library's message; there is no CPU fallback --
+if ``libpll.so`` is missing or no GPU is present, construction fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["Channel", "PllError", "load_library", "mesh_by_func", "eigen_values", "FIELDS",
+           "PRECISIONS", "PROFILES", "LIB_PATH", "HEADER_PATH"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpll.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "pll.h")
+
+FIELDS = {"u": 0, "v": 1, "w": 2, "u1": 3, "v1": 4, "w1": 5, "p": 6}
+PRECISIONS = {"fp64": 0, "fp32_mixed": 1}
+PROFILES = {"uni": 0, "poi": 1, "log": 2}
+
+
+class PllError(RuntimeError):
+    """A pll_* call returned a non-zero status."""
+
+    def __init__(self, code, what, msg):
+        super().__init__(f"{what} failed with status {code}: {msg}")
+        self.code = code
+
+
+class PllConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("re", C.c_double), ("u_ref", C.c_double), ("l_ref", C.c_double),
+        ("dt", C.c_double), ("u_crf", C.c_double),
+        ("is_forced", C.c_int * 3), ("vel_force", C.c_double * 3),
+        ("precision", C.c_int), ("pin_singular", C.c_int), ("device", C.c_int),
+    ]
+
+
+class PllValueIndex(C.Structure):
+    _fields_ = [("value", C.c_double), ("rank", C.c_int), ("ig", C.c_int), ("jg", C.c_int),
+                ("kg", C.c_int)]
+
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lib = None
+
+# name -> (restype, argtypes); every entry point include/pll.h declares.
+_SIGNATURES = {
+    "pll_abi_version": (C.c_int, []),
+    "pll_last_error": (C.c_char_p, []),
+    "pll_config_default": (None, [C.POINTER(PllConfig)]),
+    "pll_mesh_by_func": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _D]),
+    "pll_create": (C.c_int, [C.POINTER(PllConfig), _D, C.POINTER(C.c_void_p)]),
+    "pll_destroy": (C.c_int, [C.c_void_p]),
+    "pll_get_config": (C.c_int, [C.c_void_p, C.POINTER(PllConfig)]),
+    "pll_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "pll_field_upload": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "pll_field_download": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "pll_field_device_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_int)]),
+    "pll_spec_upload": (C.c_int, [C.c_void_p, _D]),
+    "pll_spec_download": (C.c_int, [C.c_void_p, _D]),
+    "pll_init_flow": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double,
+                                C.POINTER(C.c_uint32)]),
+    "pll_transform2crf": (C.c_int, [C.c_void_p, C.c_double]),
+    "pll_time_int_vel_rk1": (C.c_int, [C.c_void_p]),
+    "pll_time_int_vel_rk2": (C.c_int, [C.c_void_p]),
+    "pll_correct_vel": (C.c_int, [C.c_void_p]),
+    "pll_force_vel": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "pll_calc_rhs": (C.c_int, [C.c_void_p]),
+    "pll_poisson_execute": (C.c_int, [C.c_void_p]),
+    "pll_fft_fwd_xpen": (C.c_int, [C.c_void_p]),
+    "pll_fft_fwd_ypen": (C.c_int, [C.c_void_p]),
+    "pll_solve_trid": (C.c_int, [C.c_void_p]),
+    "pll_fft_bwd_ypen": (C.c_int, [C.c_void_p]),
+    "pll_fft_bwd_xpen": (C.c_int, [C.c_void_p]),
+    "pll_eigen_values": (C.c_int, [C.c_int, _D]),
+    "pll_update_bound_vel": (C.c_int, [C.c_void_p, C.c_int]),
+    "pll_update_bound_p": (C.c_int, [C.c_void_p]),
+    "pll_mean": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "pll_calc_max_div": (C.c_int, [C.c_void_p, C.POINTER(PllValueIndex)]),
+    "pll_calc_max_cfl": (C.c_int, [C.c_void_p, C.POINTER(PllValueIndex)]),
+    "pll_check_nan": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "pll_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "pll_synchronize": (C.c_int, [C.c_void_p]),
+    "pll_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "pll_timing_reset": (C.c_int, [C.c_void_p]),
+    "pll_timing_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "pll_timing_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_long)]),
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load libpll.so and declare every entry point; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run __graft_entry__.build() "
+                          "(make -C powerllel_channel_sycl_amd/csrc)")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = load_library().pll_last_error()
+        raise PllError(rc, what, msg.decode() if msg else "")
+
+
+def mesh_by_func(mesh_type, stretch_func, stretch_ratio, nz, lz):
+    """initMeshByFunc (src/mod_mesh.cc:95-125): dzf_global[0..nz+1]."""
+    out = np.zeros(nz + 2)
+    _check(load_library().pll_mesh_by_func(mesh_type, stretch_func, stretch_ratio, nz, lz, out),
+           "pll_mesh_by_func")
+    return out
+
+
+def eigen_values(n):
+    """get_eigen_values (src/mod_fft.cc:196-222): perm-ordered, unscaled."""
+    out = np.zeros(n)
+    _check(load_library().pll_eigen_values(n, out), "pll_eigen_values")
+    return out
+
+
+class Channel:
+    """One channel-flow problem on one GPU (the context of include/pll.h)."""
+
+    def __init__(self, nx, ny, nz, lx, ly, lz, re, dt, dzf_global, u_ref=1.0, l_ref=1.0,
+                 u_crf=0.0, is_forced=(1, 0, 0), vel_force=(1.0, 0.0, 0.0), precision="fp64",
+                 pin_singular=True, device=0):
+        lib = load_library()
+        cfg = PllConfig()
+        lib.pll_config_default(C.byref(cfg))
+        cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+        cfg.lx, cfg.ly, cfg.lz = lx, ly, lz
+        cfg.re, cfg.u_ref, cfg.l_ref = re, u_ref, l_ref
+        cfg.dt, cfg.u_crf = dt, u_crf
+        for d in range(3):
+            cfg.is_forced[d] = int(bool(is_forced[d]))
+            cfg.vel_force[d] = float(vel_force[d])
+        cfg.precision = PRECISIONS[precision]
+        cfg.pin_singular = int(bool(pin_singular))
+        cfg.device = device
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.precision = precision
+        self._h = C.c_void_p()
+        _check(lib.pll_create(C.byref(cfg), np.ascontiguousarray(dzf_global, dtype=np.float64),
+                              C.byref(self._h)), "pll_create")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_library().pll_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _call(self, name, *args):
+        _check(getattr(load_library(), name)(self._h, *args), name)
+
+    # ---- fields ---------------------------------------------------------------
+    @property
+    def field_shape(self):
+        return (self.nz + 2, self.ny + 2, self.nx + 2)
+
+    def upload(self, field, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        if arr.shape != self.field_shape:
+            raise ValueError(f"field shape {arr.shape} != {self.field_shape}")
+        self._call("pll_field_upload", FIELDS[field], arr)
+
+    def download(self, field):
+        out = np.empty(self.field_shape)
+        self._call("pll_field_download", FIELDS[field], out)
+        return out
+
+    def upload_spec(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        if arr.size != self.nx * self.ny * self.nz:
+            raise ValueError("work array must hold nx*ny*nz values")
+        self._call("pll_spec_upload", arr)
+
+    def download_spec(self):
+        out = np.empty((self.nz, self.ny, self.nx))
+        self._call("pll_spec_download", out)
+        return out
+
+    def device_bytes(self):
+        b = C.c_size_t()
+        self._call("pll_device_bytes", C.byref(b))
+        return b.value
+
+    # ---- mod_initFlow / mod_calcVel ----------------------------------------------
+    def initFlow(self, profile="poi", u0=0.0, with_noise=True, noise_intensity=0.1,
+                 seeds=(42, 43, 44)):
+        s = (C.c_uint32 * 3)(*[int(x) for x in seeds])
+        self._call("pll_init_flow", PROFILES[profile], float(u0), int(bool(with_noise)),
+                   float(noise_intensity), s)
+
+    def transform2CRF(self, vel_crf):
+        self._call("pll_transform2crf", float(vel_crf))
+
+    def timeIntVelRK1(self):
+        self._call("pll_time_int_vel_rk1")
+
+    def timeIntVelRK2(self):
+        self._call("pll_time_int_vel_rk2")
+
+    def correctVel(self):
+        self._call("pll_correct_vel")
+
+    def forceVel(self, return_mean=False):
+        if return_mean:
+            m = (C.c_double * 3)()
+            self._call("pll_force_vel", m)
+            return np.array(m[:])
+        self._call("pll_force_vel", None)
+        return None
+
+    # ---- mod_calcRHS / poisson_solver -----------------------------------------------
+    def calcRHS(self):
+        self._call("pll_calc_rhs")
+
+    def execute_poisson_solver(self):
+        self._call("pll_poisson_execute")
+
+    def execute_fft_fwd_xpen(self):
+        self._call("pll_fft_fwd_xpen")
+
+    def execute_fft_fwd_ypen(self):
+        self._call("pll_fft_fwd_ypen")
+
+    def solve_trid(self):
+        self._call("pll_solve_trid")
+
+    def execute_fft_bwd_ypen(self):
+        self._call("pll_fft_bwd_ypen")
+
+    def execute_fft_bwd_xpen(self):
+        self._call("pll_fft_bwd_xpen")
+
+    # ---- mod_updateBound -----------------------------------------------------------
+    def updateBoundVel(self, which=0):
+        self._call("pll_update_bound_vel", int(which))
+
+    def updateBoundP(self):
+        self._call("pll_update_bound_p")
+
+    # ---- mod_utils -------------------------------------------------------------------
+    def Mean(self, field="u", nx_global=None, ny_global=None):
+        out = C.c_double()
+        self._call("pll_mean", FIELDS[field], int(nx_global or self.nx), int(ny_global or self.ny),
+                   C.byref(out))
+        return out.value
+
+    def calcMaxDiv(self):
+        r = PllValueIndex()
+        self._call("pll_calc_max_div", C.byref(r))
+        return r.value, (r.ig, r.jg, r.kg)
+
+    def calcMaxCFL(self):
+        r = PllValueIndex()
+        self._call("pll_calc_max_cfl", C.byref(r))
+        return r.value, (r.ig, r.jg, r.kg)
+
+    def checkNaN(self, field="u"):
+        ok = C.c_int()
+        self._call("pll_check_nan", FIELDS[field], C.byref(ok))
+        return bool(ok.value)
+
+    # ---- main loop ---------------------------------------------------------------------
+    def step(self, nsteps=1):
+        self._call("pll_step", int(nsteps))
+
+    def synchronize(self):
+        self._call("pll_synchronize")
+
+    # ---- timers --------------------------------------------------------------------------
+    def timing(self, on=True):
+        self._call("pll_timing_enable", int(bool(on)))
+
+    def timing_reset(self):
+        self._call("pll_timing_reset")
+
+    def timing_read(self):
+        """{kernel name: (total ms, launches)} of the HIP-event timers."""
+        n = C.c_int()
+        self._call("pll_timing_count", C.byref(n))
+        out = {}
+        for k in range(n.value):
+            name, ms, cnt = C.c_char_p(), C.c_double(), C.c_long()
+            self._call("pll_timing_read", k, C.byref(name), C.byref(ms), C.byref(cnt))
+            out[name.value.decode()] = (ms.value, cnt.value)
+        return out

@@ -1,0 +1,762 @@
+// k_stencil.hip -- momentum RK stages, Poisson RHS, projection, forcing, the
+// boundary/halo update, the bulk-velocity reduction, the initial condition and
+// the runtime checks.  All kernels are templated on the field type T (double
+// for the FP64 path, float for the FP32-mixed path).
+//
+// FP64 parity: the library is compiled with -ffp-contract=off and every
+// expression below keeps the association of the reference formula it cites,
+// so an FP64 launch reproduces the reference arithmetic bit for bit.
+#include "pll_internal.hpp"
+
+#include <cmath>
+
+namespace pll {
+
+namespace {
+
+constexpr int kTx = 64;  // threads along x (one wavefront)
+constexpr int kTy = 4;   // rows along y per block
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Momentum tendency F = visc - conv of the three staggered components at cell
+// (i,j,k), src/mod_calcVel.cc:61-125 (stage 1) and :174-238 (stage 2).  Faces:
+// u at (i+1/2, j, k), v at (i, j+1/2, k), w at (i, j, k+1/2).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void tendency(const T* __restrict__ u, const T* __restrict__ v,
+                                         const T* __restrict__ w, const MeshT<T>& m, long c,
+                                         long ys, long zs, int k, T dxi, T dyi, T rei, T ucrf,
+                                         T& tu, T& tv, T& tw) {
+  const T quarter = T(0.25), half = T(0.5);
+  const T dzf_k = m.dzf[k], dzf_p = m.dzf[k + 1], dzf_m = m.dzf[k - 1];
+  const T dci_k = m.dzc_inv[k], dci_m = m.dzc_inv[k - 1];
+  const T dfi_k = m.dzf_inv[k], dfi_p = m.dzf_inv[k + 1];
+
+  // ---- x-momentum at the u face ----
+  {
+    const T q0 = u[c], qe = u[c + 1], qw = u[c - 1], qn = u[c + ys], qs = u[c - ys];
+    const T qt = u[c + zs], qb = u[c - zs];
+    const T ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
+    const T at = (q0 * dzf_p + qt * dzf_k) * dci_k;
+    const T ab = (q0 * dzf_m + qb * dzf_k) * dci_m;
+    const T vn = v[c] + v[c + 1];
+    const T vs = v[c - ys] + v[c - ys + 1];
+    const T wt = w[c] + w[c + 1];
+    const T wb = w[c - zs] + w[c - zs + 1];
+    const T ge = (qe - q0) * dxi, gw = (q0 - qw) * dxi;
+    const T gn = (qn - q0) * dyi, gs = (q0 - qs) * dyi;
+    const T gt = (qt - q0) * dci_k, gb = (q0 - qb) * dci_m;
+    T conv = quarter * ((ae * ae - aw * aw) * dxi + (an * vn - as * vs) * dyi +
+                        (at * wt - ab * wb) * dfi_k);
+    conv = conv + half * ucrf * (ae - aw) * dxi;
+    const T visc = ((ge - gw) * dxi + (gn - gs) * dyi + (gt - gb) * dfi_k) * rei;
+    tu = visc - conv;
+  }
+  // ---- y-momentum at the v face ----
+  {
+    const T q0 = v[c], qe = v[c + 1], qw = v[c - 1], qn = v[c + ys], qs = v[c - ys];
+    const T qt = v[c + zs], qb = v[c - zs];
+    const T ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
+    const T at = (q0 * dzf_p + qt * dzf_k) * dci_k;
+    const T ab = (q0 * dzf_m + qb * dzf_k) * dci_m;
+    const T ue = u[c] + u[c + ys];
+    const T uw = u[c - 1] + u[c - 1 + ys];
+    const T wt = w[c] + w[c + ys];
+    const T wb = w[c - zs] + w[c - zs + ys];
+    const T ge = (qe - q0) * dxi, gw = (q0 - qw) * dxi;
+    const T gn = (qn - q0) * dyi, gs = (q0 - qs) * dyi;
+    const T gt = (qt - q0) * dci_k, gb = (q0 - qb) * dci_m;
+    T conv = quarter * ((ue * ae - uw * aw) * dxi + (an * an - as * as) * dyi +
+                        (wt * at - wb * ab) * dfi_k);
+    conv = conv + half * ucrf * (ae - aw) * dxi;
+    const T visc = ((ge - gw) * dxi + (gn - gs) * dyi + (gt - gb) * dfi_k) * rei;
+    tv = visc - conv;
+  }
+  // ---- z-momentum at the w face ----
+  {
+    const T q0 = w[c], qe = w[c + 1], qw = w[c - 1], qn = w[c + ys], qs = w[c - ys];
+    const T qt = w[c + zs], qb = w[c - zs];
+    const T ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
+    const T at = q0 + qt, ab = q0 + qb;
+    const T ue = (u[c] * dzf_p + u[c + zs] * dzf_k) * dci_k;
+    const T uw = (u[c - 1] * dzf_p + u[c - 1 + zs] * dzf_k) * dci_k;
+    const T vn = (v[c] * dzf_p + v[c + zs] * dzf_k) * dci_k;
+    const T vs = (v[c - ys] * dzf_p + v[c - ys + zs] * dzf_k) * dci_k;
+    const T ge = (qe - q0) * dxi, gw = (q0 - qw) * dxi;
+    const T gn = (qn - q0) * dyi, gs = (q0 - qs) * dyi;
+    const T gt = (qt - q0) * dfi_p, gb = (q0 - qb) * dfi_k;
+    T conv = quarter * ((ue * ae - uw * aw) * dxi + (vn * an - vs * as) * dyi +
+                        (at * at - ab * ab) * dci_k);
+    conv = conv + half * ucrf * (ae - aw) * dxi;
+    const T visc = ((ge - gw) * dxi + (gn - gs) * dyi + (gt - gb) * dci_k) * rei;
+    tw = visc - conv;
+  }
+}
+
+// Stage 1: d = s + dt*F(s).  Stage 2: d = (s + dt*F(s) + d)/2.
+// One thread per (i,j) column, marching over a chunk of k planes so the
+// vertical neighbours of consecutive iterations stay in the CU's caches.
+template <typename T, int STAGE>
+__global__ __launch_bounds__(kTx* kTy) void k_rk(Geom g, const T* __restrict__ su,
+                                                 const T* __restrict__ sv,
+                                                 const T* __restrict__ sw, T* __restrict__ du,
+                                                 T* __restrict__ dv, T* __restrict__ dw,
+                                                 MeshT<T> m, T dxi, T dyi, T rei, T dt, T ucrf,
+                                                 int kchunk) {
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  if (i > g.nx || j > g.ny) return;
+  const int k0 = blockIdx.z * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  const long ys = g.sx, zs = g.sxy;
+  for (int k = k0; k <= k1; ++k) {
+    const long c = g.idx(i, j, k);
+    T tu, tv, tw;
+    tendency<T>(su, sv, sw, m, c, ys, zs, k, dxi, dyi, rei, ucrf, tu, tv, tw);
+    if (STAGE == 1) {
+      du[c] = su[c] + dt * tu;
+      dv[c] = sv[c] + dt * tv;
+      dw[c] = sw[c] + dt * tw;
+    } else {
+      du[c] = (su[c] + dt * tu + du[c]) * T(0.5);
+      dv[c] = (sv[c] + dt * tv + dv[c]) * T(0.5);
+      dw[c] = (sw[c] + dt * tw + dw[c]) * T(0.5);
+    }
+  }
+}
+
+// calcRHS, src/mod_calcRHS.cc:27-29.
+template <typename T>
+__global__ __launch_bounds__(kTx* kTy) void k_calc_rhs(Geom g, const T* __restrict__ u,
+                                                       const T* __restrict__ v,
+                                                       const T* __restrict__ w, T* __restrict__ p,
+                                                       const T* __restrict__ dzf_inv, T dti,
+                                                       T dtidxi, T dtidyi, int kchunk) {
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  if (i > g.nx || j > g.ny) return;
+  const int k0 = blockIdx.z * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  for (int k = k0; k <= k1; ++k) {
+    const long c = g.idx(i, j, k);
+    p[c] = (u[c] - u[c - 1]) * dtidxi + (v[c] - v[c - g.sx]) * dtidyi +
+           (w[c] - w[c - g.sxy]) * dti * dzf_inv[k];
+  }
+}
+
+// correctVel, src/mod_calcVel.cc:272-274.
+template <typename T>
+__global__ __launch_bounds__(kTx* kTy) void k_correct(Geom g, const T* __restrict__ p,
+                                                      T* __restrict__ u, T* __restrict__ v,
+                                                      T* __restrict__ w,
+                                                      const T* __restrict__ dzc_inv, T dtdxi,
+                                                      T dtdyi, T dt, int kchunk) {
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  if (i > g.nx || j > g.ny) return;
+  const int k0 = blockIdx.z * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  for (int k = k0; k <= k1; ++k) {
+    const long c = g.idx(i, j, k);
+    const T p0 = p[c];
+    u[c] -= (p[c + 1] - p0) * dtdxi;
+    v[c] -= (p[c + g.sx] - p0) * dtdyi;
+    w[c] -= (p[c + g.sxy] - p0) * dt * dzc_inv[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bulk mean (Mean, src/mod_utils.cc:30-51) in FP64: per-block partial sums of
+// f*dzflzi(k) over a chunk of rows of one plane, then one block combines the
+// partials in a fixed order, so the result is the same on every run.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double s, double* sh) {
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_mean_partial(Geom g, const T* __restrict__ f,
+                                                              const T* __restrict__ dzflzi,
+                                                              double* __restrict__ partial,
+                                                              int rows_per_chunk) {
+  __shared__ double sh[kRedThreads];
+  const int k = blockIdx.y + 1;
+  const int j0 = blockIdx.x * rows_per_chunk;
+  const int j1 = min(j0 + rows_per_chunk, g.ny);
+  const double wk = (double)dzflzi[k];
+  double s = 0.0;
+  for (int j = j0; j < j1; ++j) {
+    const T* row = f + g.idx(1, j + 1, k);
+    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) s += (double)row[i] * wk;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// out[0] = sum(partial)/nxg/nyg; when force_slot >= 0 also stores
+// out[force_slot] = vel_force - mean (forceVel, src/mod_calcVel.cc:297).
+__global__ __launch_bounds__(kRedThreads) void k_mean_final(const double* __restrict__ partial,
+                                                            long n, double nxg, double nyg,
+                                                            double* out, double vel_force,
+                                                            double* force_out) {
+  __shared__ double sh[kRedThreads];
+  double s = 0.0;
+  for (long t = threadIdx.x; t < n; t += blockDim.x) s += partial[t];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    const double mean = s / nxg / nyg;
+    out[0] = mean;
+    if (force_out) force_out[0] = vel_force - mean;
+  }
+}
+
+// forceVel update (src/mod_calcVel.cc:299-315): interior += force.
+template <typename T>
+__global__ __launch_bounds__(kTx* kTy) void k_add_force(Geom g, T* __restrict__ f,
+                                                        const double* __restrict__ force,
+                                                        int kchunk) {
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  if (i > g.nx || j > g.ny) return;
+  const T fv = (T)force[0];
+  const int k0 = blockIdx.z * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  for (int k = k0; k <= k1; ++k) {
+    const long c = g.idx(i, j, k);
+    f[c] += fv;
+  }
+}
+
+// transform2CRF and other uniform shifts: interior += value.
+template <typename T>
+__global__ __launch_bounds__(kTx* kTy) void k_add_value(Geom g, T* __restrict__ f, T value,
+                                                        int kchunk) {
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  if (i > g.nx || j > g.ny) return;
+  const int k0 = blockIdx.z * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  for (int k = k0; k <= k1; ++k) f[g.idx(i, j, k)] += value;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary update at one GPU.  The reference's sequence -- y self-exchange,
+// x periodic copies over the whole y/z extent, then the z walls over the whole
+// x/y extent (src/mod_updateBound.cc:229-296) -- leaves every halo cell equal
+// to a closed form of interior values:
+//   side halo (i or j in {0, n+1}, 1 <= k <= nz): f(wrap i, wrap j, k)
+//   z planes: centred u, v: 2*bc - f(wrap i, wrap j, 1|nz); face w: bc on
+//   planes 0, nz and nz+1 (src/mod_updateBound.cc:75-144); p: zero gradient.
+// Each halo cell is written once from interior cells, so one launch covers
+// all of them with no ordering hazard.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrap(int i, int n) { return i == 0 ? n : (i == n + 1 ? 1 : i); }
+
+// Perimeter cell q of an (nx+2)x(ny+2) plane -> (i, j).
+__device__ __forceinline__ void perimeter(int q, int nx, int ny, int& i, int& j) {
+  const int xl = nx + 2;
+  if (q < xl) {
+    i = q;
+    j = 0;
+  } else if (q < 2 * xl) {
+    i = q - xl;
+    j = ny + 1;
+  } else {
+    const int r = q - 2 * xl;  // rows 1..ny, two cells each
+    j = r / 2 + 1;
+    i = (r & 1) ? nx + 1 : 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_vel(Geom g, T* __restrict__ u, T* __restrict__ v,
+                                                T* __restrict__ w, T bc_u) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nper = 2 * (g.nx + 2) + 2 * g.ny;
+  const long nside = (long)nper * g.nz;
+  const long nplane = (long)(g.nx + 2) * (g.ny + 2);
+  const T zero = T(0), two = T(2);
+  if (t < nside) {
+    const int k = (int)(t / nper) + 1;
+    int i, j;
+    perimeter((int)(t % nper), g.nx, g.ny, i, j);
+    const long d = g.idx(i, j, k);
+    const long s = g.idx(wrap(i, g.nx), wrap(j, g.ny), k);
+    u[d] = u[s];
+    v[d] = v[s];
+    w[d] = (k == g.nz) ? zero : w[s];
+  } else if (t < nside + nplane) {
+    const long q = t - nside;
+    const int i = (int)(q % (g.nx + 2));
+    const int j = (int)(q / (g.nx + 2));
+    const int wi = wrap(i, g.nx), wj = wrap(j, g.ny);
+    u[g.idx(i, j, 0)] = two * bc_u - u[g.idx(wi, wj, 1)];
+    u[g.idx(i, j, g.nz + 1)] = two * bc_u - u[g.idx(wi, wj, g.nz)];
+    v[g.idx(i, j, 0)] = two * zero - v[g.idx(wi, wj, 1)];
+    v[g.idx(i, j, g.nz + 1)] = two * zero - v[g.idx(wi, wj, g.nz)];
+    w[g.idx(i, j, 0)] = zero;
+    w[g.idx(i, j, g.nz)] = zero;
+    w[g.idx(i, j, g.nz + 1)] = zero;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bc_p(Geom g, T* __restrict__ p) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nper = 2 * (g.nx + 2) + 2 * g.ny;
+  const long nside = (long)nper * g.nz;
+  const long nplane = (long)(g.nx + 2) * (g.ny + 2);
+  if (t < nside) {
+    const int k = (int)(t / nper) + 1;
+    int i, j;
+    perimeter((int)(t % nper), g.nx, g.ny, i, j);
+    p[g.idx(i, j, k)] = p[g.idx(wrap(i, g.nx), wrap(j, g.ny), k)];
+  } else if (t < nside + nplane) {
+    const long q = t - nside;
+    const int i = (int)(q % (g.nx + 2));
+    const int j = (int)(q / (g.nx + 2));
+    const int wi = wrap(i, g.nx), wj = wrap(j, g.ny);
+    p[g.idx(i, j, 0)] = p[g.idx(wi, wj, 1)];
+    p[g.idx(i, j, g.nz + 1)] = p[g.idx(wi, wj, g.nz)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Initial condition (src/mod_initFlow.cc:47-102): u = profile(k) + noise,
+// v = w = noise.  Noise: std::minstd_rand (a = 48271, m = 2^31-1) seeded with
+// seed and advanced L+1 steps, L the interior linear index.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mulmod31(uint64_t a, uint64_t b) {
+  return (a * b) % 2147483647ULL;
+}
+
+__device__ double noise_uniform(uint32_t seed, uint64_t offset) {
+  uint64_t x = seed % 2147483647ULL;
+  if (x == 0) x = 1;
+  uint64_t base = 48271ULL, e = offset + 1, r = 1;
+  while (e) {
+    if (e & 1) r = mulmod31(r, base);
+    base = mulmod31(base, base);
+    e >>= 1;
+  }
+  x = mulmod31(x, r);
+  return (double)(x - 1) / 2147483646.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTx* kTy) void k_init(Geom g, const double* __restrict__ prof,
+                                                   T* __restrict__ u, T* __restrict__ v,
+                                                   T* __restrict__ w, uint32_t s0, uint32_t s1,
+                                                   uint32_t s2, int with_noise, double noise,
+                                                   int kchunk) {
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  if (i > g.nx || j > g.ny) return;
+  const int k0 = blockIdx.z * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  for (int k = k0; k <= k1; ++k) {
+    const long c = g.idx(i, j, k);
+    double uu = prof[k], vv = 0.0, ww = 0.0;
+    if (with_noise) {
+      const uint64_t L = ((uint64_t)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1);
+      uu += 2.0 * (noise_uniform(s0, L) - 0.5) * noise;
+      vv += 2.0 * (noise_uniform(s1, L) - 0.5) * noise;
+      ww += 2.0 * (noise_uniform(s2, L) - 0.5) * noise;
+    }
+    u[c] = (T)uu;
+    v[c] = (T)vv;
+    w[c] = (T)ww;
+  }
+}
+
+// u *= 1/(mean*u_ref) when mean != 0 (src/mod_initFlow.cc:91-100).
+template <typename T>
+__global__ __launch_bounds__(kTx* kTy) void k_scale_inv_mean(Geom g, T* __restrict__ f,
+                                                             const double* __restrict__ mean,
+                                                             double u_ref, int kchunk) {
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  if (i > g.nx || j > g.ny) return;
+  const double m = mean[0];
+  if (m == 0.0) return;
+  const double mult = 1.0 / (m * u_ref);
+  const int k0 = blockIdx.z * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  for (int k = k0; k <= k1; ++k) {
+    const long c = g.idx(i, j, k);
+    f[c] = (T)((double)f[c] * mult);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Runtime checks (src/mod_utils.cc:77-217).  Max with location: a candidate
+// replaces the running best only when strictly larger (NaN never does), and
+// ties between threads go to the smaller linear index, which reproduces the
+// reference's k-j-i loop's first maximum.
+// ---------------------------------------------------------------------------
+struct ValIdx {
+  double v;
+  long l;
+};
+
+__device__ __forceinline__ bool better(double va, long la, double vb, long lb) {
+  return va > vb || (va == vb && la < lb);
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(kRedThreads) void k_max_partial(Geom g, const T* __restrict__ u,
+                                                             const T* __restrict__ v,
+                                                             const T* __restrict__ w,
+                                                             const T* __restrict__ dzf_inv,
+                                                             T dxi, T dyi, ValIdx* out) {
+  __shared__ double sv[kRedThreads];
+  __shared__ long sl[kRedThreads];
+  const long n = (long)g.nx * g.ny * g.nz;
+  double best = 0.0;
+  long bl = 0;
+  for (long L = (long)blockIdx.x * blockDim.x + threadIdx.x; L < n;
+       L += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(L % g.nx) + 1;
+    const int j = (int)((L / g.nx) % g.ny) + 1;
+    const int k = (int)(L / ((long)g.nx * g.ny)) + 1;
+    const long c = g.idx(i, j, k);
+    T val;
+    if (KIND == 0) {  // divergence, src/mod_utils.cc:177-179
+      val = fabs((u[c] - u[c - 1]) * dxi + (v[c] - v[c - g.sx]) * dyi +
+                 (w[c] - w[c - g.sxy]) * dzf_inv[k]);
+    } else {  // CFL, src/mod_utils.cc:121-123
+      val = fabs((u[c] + u[c - 1]) * T(0.5)) * dxi + fabs((v[c] + v[c - g.sx]) * T(0.5)) * dyi +
+            fabs((w[c] + w[c - g.sxy]) * T(0.5)) * dzf_inv[k];
+    }
+    const double dv = (double)val;
+    if (dv > best) {
+      best = dv;
+      bl = L;
+    }
+  }
+  sv[threadIdx.x] = best;
+  sl[threadIdx.x] = bl;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h &&
+        better(sv[threadIdx.x + h], sl[threadIdx.x + h], sv[threadIdx.x], sl[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + h];
+      sl[threadIdx.x] = sl[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = ValIdx{sv[0], sl[0]};
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_max_final(const ValIdx* __restrict__ in, int n,
+                                                           ValIdx* out) {
+  __shared__ double sv[kRedThreads];
+  __shared__ long sl[kRedThreads];
+  double best = 0.0;
+  long bl = 0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x)
+    if (better(in[t].v, in[t].l, best, bl)) {
+      best = in[t].v;
+      bl = in[t].l;
+    }
+  sv[threadIdx.x] = best;
+  sl[threadIdx.x] = bl;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h &&
+        better(sv[threadIdx.x + h], sl[threadIdx.x + h], sv[threadIdx.x], sl[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + h];
+      sl[threadIdx.x] = sl[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = ValIdx{sv[0], sl[0]};
+}
+
+// NaN count over the whole halo-padded array.
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_nan_partial(const T* __restrict__ f, long n,
+                                                             double* out) {
+  __shared__ double sh[kRedThreads];
+  double cnt = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x)
+    if (isnan(f[t])) cnt += 1.0;
+  cnt = block_sum(cnt, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// host-side helpers
+// ---------------------------------------------------------------------------
+inline int kchunk_for(const pll_context* c) {
+  // enough blocks to fill 256 CUs several times over, k-chunks of >= 4 planes
+  const long cols = (long)((c->g.nx + kTx - 1) / kTx) * ((c->g.ny + kTy - 1) / kTy);
+  long want = 4096 / (cols > 0 ? cols : 1);
+  if (want < 1) want = 1;
+  long kc = (c->g.nz + want - 1) / want;
+  if (kc < 4) kc = 4;
+  if (kc > c->g.nz) kc = c->g.nz;
+  return (int)kc;
+}
+
+inline dim3 col_grid(const pll_context* c, int kc) {
+  return dim3((c->g.nx + kTx - 1) / kTx, (c->g.ny + kTy - 1) / kTy, (c->g.nz + kc - 1) / kc);
+}
+
+template <typename T>
+inline MeshT<T> mesh_of(const pll_context* c) {
+  const T* base = static_cast<const T*>(c->d_mesh);
+  const long n = c->g.nz + 2;
+  return MeshT<T>{base, base + n, base + 2 * n, base + 3 * n};
+}
+
+template <typename T>
+inline T* F(pll_context* c, int f) {
+  return static_cast<T*>(c->fld[f]);
+}
+
+template <typename T>
+int rk_t(pll_context* c, int stage) {
+  const int kc = kchunk_for(c);
+  const dim3 grid = col_grid(c, kc), block(kTx, kTy);
+  const MeshT<T> m = mesh_of<T>(c);
+  if (stage == 1) {
+    TimerScope ts(c, K_RK1);
+    hipLaunchKernelGGL((k_rk<T, 1>), grid, block, 0, c->stream, c->g, F<T>(c, PLL_U),
+                       F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1), F<T>(c, PLL_V1),
+                       F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv, (T)c->re_inv, (T)c->dt,
+                       (T)c->u_crf, kc);
+  } else {
+    TimerScope ts(c, K_RK2);
+    hipLaunchKernelGGL((k_rk<T, 2>), grid, block, 0, c->stream, c->g, F<T>(c, PLL_U1),
+                       F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U), F<T>(c, PLL_V),
+                       F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv, (T)c->re_inv, (T)c->dt,
+                       (T)c->u_crf, kc);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int rhs_t(pll_context* c) {
+  const int kc = kchunk_for(c);
+  const double dti = 1.0 / c->dt;  // src/mod_calcRHS.cc:11-13
+  const double dtidxi = dti * c->dx_inv;
+  const double dtidyi = dti * c->dy_inv;
+  TimerScope ts(c, K_RHS);
+  hipLaunchKernelGGL((k_calc_rhs<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+                     F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_P),
+                     mesh_of<T>(c).dzf_inv, (T)dti, (T)dtidxi, (T)dtidyi, kc);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int correct_t(pll_context* c) {
+  const int kc = kchunk_for(c);
+  const double dtdxi = c->dt * c->dx_inv;  // src/mod_calcVel.cc:256-258
+  const double dtdyi = c->dt * c->dy_inv;
+  TimerScope ts(c, K_CORRECT);
+  hipLaunchKernelGGL((k_correct<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+                     F<T>(c, PLL_P), F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W),
+                     mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt, kc);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int mean_t(pll_context* c, int field, double nxg, double nyg, double* d_out, double vel_force,
+           double* d_force) {
+  const int chunks = c->mean_chunks;
+  const int rpc = (c->g.ny + chunks - 1) / chunks;
+  {
+    TimerScope ts(c, K_MEAN_PARTIAL);
+    hipLaunchKernelGGL((k_mean_partial<T>), dim3(chunks, c->g.nz), dim3(kRedThreads), 0,
+                       c->stream, c->g, F<T>(c, field), mesh_of<T>(c).dzflzi, c->d_partial, rpc);
+  }
+  {
+    TimerScope ts(c, K_MEAN_FINAL);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kRedThreads), 0, c->stream, c->d_partial,
+                       (long)chunks * c->g.nz, nxg, nyg, d_out, vel_force, d_force);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int force_t(pll_context* c) {
+  const int kc = kchunk_for(c);
+  for (int d = 0; d < 3; ++d) {
+    if (!c->is_forced[d]) continue;
+    int rc = mean_t<T>(c, PLL_U + d, (double)c->g.nx, (double)c->g.ny, c->d_scalar + d,
+                       c->vel_force[d], c->d_scalar + 3 + d);
+    if (rc) return rc;
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (!c->is_forced[d]) continue;
+    TimerScope ts(c, K_FORCE);
+    hipLaunchKernelGGL((k_add_force<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+                       F<T>(c, PLL_U + d), c->d_scalar + 3 + d, kc);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+inline long bc_threads(const pll_context* c) {
+  const long nper = 2L * (c->g.nx + 2) + 2L * c->g.ny;
+  return nper * c->g.nz + (long)(c->g.nx + 2) * (c->g.ny + 2);
+}
+
+template <typename T>
+int bc_vel_t(pll_context* c, int which) {
+  const long n = bc_threads(c);
+  const int base = which ? PLL_U1 : PLL_U;
+  const double bc_u = 0.0 - c->u_crf;  // src/mod_updateBound.cc:80
+  TimerScope ts(c, K_BC_VEL);
+  hipLaunchKernelGGL((k_bc_vel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->g, F<T>(c, base), F<T>(c, base + 1), F<T>(c, base + 2), (T)bc_u);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int bc_p_t(pll_context* c) {
+  const long n = bc_threads(c);
+  TimerScope ts(c, K_BC_P);
+  hipLaunchKernelGGL((k_bc_p<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->g, F<T>(c, PLL_P));
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int add_scalar_t(pll_context* c, int field, double value) {
+  const int kc = kchunk_for(c);
+  TimerScope ts(c, K_MISC);
+  hipLaunchKernelGGL((k_add_value<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+                     F<T>(c, field), (T)value, kc);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int init_t(pll_context* c, const double* d_prof, int with_noise, double noise,
+           const uint32_t seeds[3]) {
+  const int kc = kchunk_for(c);
+  TimerScope ts(c, K_INIT);
+  hipLaunchKernelGGL((k_init<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g, d_prof,
+                     F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), seeds[0], seeds[1],
+                     seeds[2], with_noise, noise, kc);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T>
+int scale_inv_mean_t(pll_context* c, int field, const double* d_mean, double u_ref) {
+  const int kc = kchunk_for(c);
+  TimerScope ts(c, K_INIT);
+  hipLaunchKernelGGL((k_scale_inv_mean<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+                     F<T>(c, field), d_mean, u_ref, kc);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T, int KIND>
+int max_t(pll_context* c, pll_value_index* out) {
+  ValIdx* red = static_cast<ValIdx*>(c->d_red);
+  {
+    TimerScope ts(c, K_CHECK);
+    hipLaunchKernelGGL((k_max_partial<T, KIND>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+                       c->stream, c->g, F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W),
+                       mesh_of<T>(c).dzf_inv, (T)c->dx_inv, (T)c->dy_inv, red);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kRedThreads), 0, c->stream, red, kRedBlocks,
+                       red + kRedBlocks);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  ValIdx h;
+  PLL_HIP_TRY(hipMemcpyAsync(&h, red + kRedBlocks, sizeof(ValIdx), hipMemcpyDeviceToHost,
+                             c->stream));
+  PLL_HIP_TRY(hipStreamSynchronize(c->stream));
+  const long nxy = (long)c->g.nx * c->g.ny;
+  out->value = (KIND == 1) ? h.v * (double)(T)c->dt : h.v;
+  out->rank = 0;
+  out->ig = (int)(h.l % c->g.nx) + 1;
+  out->jg = (int)((h.l / c->g.nx) % c->g.ny) + 1;
+  out->kg = (int)(h.l / nxy) + 1;
+  return PLL_OK;
+}
+
+template <typename T>
+int nan_t(pll_context* c, int field, int* passed) {
+  double* red = static_cast<double*>(c->d_red);
+  {
+    TimerScope ts(c, K_CHECK);
+    hipLaunchKernelGGL((k_nan_partial<T>), dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream,
+                       F<T>(c, field), c->nfield, red);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kRedThreads), 0, c->stream, red,
+                       (long)kRedBlocks, 1.0, 1.0, red + kRedBlocks, 0.0, (double*)nullptr);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  double cnt = 0.0;
+  PLL_HIP_TRY(hipMemcpyAsync(&cnt, red + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+  PLL_HIP_TRY(hipStreamSynchronize(c->stream));
+  *passed = (cnt == 0.0) ? 1 : 0;
+  return PLL_OK;
+}
+
+}  // namespace
+
+// ---- dispatch on the context precision ------------------------------------------
+#define PLL_DISPATCH(c, call_fp64, call_fp32) \
+  ((c)->prec == PLL_FP64 ? (call_fp64) : (call_fp32))
+
+int launch_rk(pll_context* c, int stage) {
+  return PLL_DISPATCH(c, rk_t<double>(c, stage), rk_t<float>(c, stage));
+}
+int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
+int launch_correct_vel(pll_context* c) {
+  return PLL_DISPATCH(c, correct_t<double>(c), correct_t<float>(c));
+}
+int launch_mean(pll_context* c, int field, double nxg, double nyg, double* d_out) {
+  return PLL_DISPATCH(c, mean_t<double>(c, field, nxg, nyg, d_out, 0.0, nullptr),
+                      mean_t<float>(c, field, nxg, nyg, d_out, 0.0, nullptr));
+}
+int launch_force(pll_context* c) { return PLL_DISPATCH(c, force_t<double>(c), force_t<float>(c)); }
+int launch_bc_vel(pll_context* c, int which) {
+  return PLL_DISPATCH(c, bc_vel_t<double>(c, which), bc_vel_t<float>(c, which));
+}
+int launch_bc_p(pll_context* c) { return PLL_DISPATCH(c, bc_p_t<double>(c), bc_p_t<float>(c)); }
+int launch_add_scalar(pll_context* c, int field, double value) {
+  return PLL_DISPATCH(c, add_scalar_t<double>(c, field, value),
+                      add_scalar_t<float>(c, field, value));
+}
+int launch_init_flow(pll_context* c, const double* d_prof, int with_noise, double noise,
+                     const uint32_t seeds[3]) {
+  return PLL_DISPATCH(c, init_t<double>(c, d_prof, with_noise, noise, seeds),
+                      init_t<float>(c, d_prof, with_noise, noise, seeds));
+}
+int launch_scale_by_inverse_mean(pll_context* c, int field, const double* d_mean, double u_ref) {
+  return PLL_DISPATCH(c, scale_inv_mean_t<double>(c, field, d_mean, u_ref),
+                      scale_inv_mean_t<float>(c, field, d_mean, u_ref));
+}
+int launch_max_div(pll_context* c, pll_value_index* out) {
+  return PLL_DISPATCH(c, (max_t<double, 0>(c, out)), (max_t<float, 0>(c, out)));
+}
+int launch_max_cfl(pll_context* c, pll_value_index* out) {
+  return PLL_DISPATCH(c, (max_t<double, 1>(c, out)), (max_t<float, 1>(c, out)));
+}
+int launch_check_nan(pll_context* c, int field, int* passed) {
+  return PLL_DISPATCH(c, nan_t<double>(c, field, passed), nan_t<float>(c, field, passed));
+}
+
+}  // namespace pll

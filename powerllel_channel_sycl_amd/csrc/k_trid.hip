@@ -1,0 +1,134 @@
+// k_trid.hip -- the wall-normal tridiagonal solve of the pressure Poisson
+// equation for every (kx, ky) mode (ssolve_trid, src/mod_poisson_solver.cc:317-357).
+//
+// The reference stores the forward-eliminated pivots as a full nx*ny*nz
+// array (src/mod_poisson_solver.cc:255-282).  The pivots of a column depend
+// only on its eigenvalue lambda_x(i) + lambda_y(j), and the perm packing gives
+// the real and imaginary parts of a mode the same eigenvalue, so four columns
+// share each pivot column.  The table here keeps one pivot column per
+// distinct pair: (nx/2+1)*(ny/2+1)*nz values, built on the host with the
+// reference's arithmetic, so every pivot is bit-identical to the reference's.
+//
+// One thread per (i,j) column, coalesced over i, sequential in k.  The
+// arithmetic is FP64 in both precision modes (the FP32-mixed path stores the
+// work array in FP32 but eliminates in FP64, which covers the kx=ky=0 and the
+// near-singular low-wavenumber columns).
+#include "pll_internal.hpp"
+
+#include <cmath>
+
+namespace pll {
+
+namespace {
+
+constexpr int kTx = 64;
+constexpr int kTy = 4;
+
+// perm index -> compact eigenvalue index: 0 -> 0, 1 -> 1 (Nyquist), 2m/2m+1 -> m+1.
+__host__ __device__ __forceinline__ int compact_index(int i) { return i <= 1 ? i : (i >> 1) + 1; }
+
+template <typename T>
+__global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __restrict__ var,
+                                                   const double* __restrict__ a,
+                                                   const double* __restrict__ c,
+                                                   const double* __restrict__ piv, int cxn,
+                                                   int cyn) {
+  const int i = blockIdx.x * kTx + threadIdx.x;
+  const int j = blockIdx.y * kTy + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const long nxy = (long)nx * ny;
+  const long P = (long)cxn * cyn;
+  const double* pv = piv + (long)compact_index(j) * cxn + compact_index(i);
+  T* col = var + (long)j * nx + i;
+  // forward elimination, src/mod_poisson_solver.cc:334-339
+  double prev = (double)col[0];
+  for (int k = 1; k < nz; ++k) {
+    const double bv = pv[(k - 1) * P];
+    const double a_tmp = a[k] / bv;
+    const double cur = (double)col[k * nxy] - a_tmp * prev;
+    col[k * nxy] = (T)cur;
+    prev = cur;
+  }
+  // last row, src/mod_poisson_solver.cc:341-346
+  const double b_val = pv[(nz - 1) * P];
+  prev = (b_val != 0) ? prev / b_val : 0.0;
+  col[(nz - 1) * nxy] = (T)prev;
+  // back substitution, src/mod_poisson_solver.cc:348-354
+  for (int k = nz - 2; k >= 0; --k) {
+    const double bv = pv[k * P];
+    const double cur = ((double)col[k * nxy] - c[k] * prev) / bv;
+    col[k * nxy] = (T)cur;
+    prev = cur;
+  }
+}
+
+}  // namespace
+
+// init_poisson_solver's eigenvalue, coefficient and pivot set-up
+// (src/mod_poisson_solver.cc:230-282, src/mod_fft.cc:196-222) at one GPU,
+// boundary types "PP","PP","NN" (src/main.cc:55-57).
+int trid_setup(pll_context* c) {
+  const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+  std::vector<double> lx(nx), ly(ny);
+  pll_eigen_values(nx, lx.data());
+  for (int i = 0; i < nx; ++i) lx[i] /= (c->dx * c->dx);
+  pll_eigen_values(ny, ly.data());
+  for (int j = 0; j < ny; ++j) ly[j] /= (c->dy * c->dy);
+
+  // set_trid_coeff('c', "NN") with dzf0[k] = dzf_global(k), k = 0..nz+1
+  // (src/mod_poisson_solver.cc:35-98,258-264)
+  const double* dzf0 = c->dzf_global.data();
+  std::vector<double> a(nz), b(nz), cc(nz);
+  for (int k = 0; k < nz; ++k) {
+    a[k] = 2.0 / (dzf0[k + 1] * (dzf0[k] + dzf0[k + 1]));
+    cc[k] = 2.0 / (dzf0[k + 1] * (dzf0[k + 2] + dzf0[k + 1]));
+  }
+  for (int k = 0; k < nz; ++k) b[k] = -a[k] - cc[k];
+  const double factor = 1.0;  // 'N'
+  b[0] += factor * a[0];
+  a[0] *= std::fabs(factor) - 1.0;
+  b[nz - 1] += factor * cc[nz - 1];
+  cc[nz - 1] *= std::fabs(factor) - 1.0;
+
+  c->cxn = nx / 2 + 1;
+  c->cyn = ny / 2 + 1;
+  const long P = (long)c->cxn * c->cyn;
+  std::vector<double> piv((size_t)P * nz);
+  auto rep = [](int ci) { return ci <= 1 ? ci : 2 * (ci - 1); };  // a perm index of that class
+  for (int cy = 0; cy < c->cyn; ++cy)
+    for (int cx = 0; cx < c->cxn; ++cx) {
+      const double lxy = lx[rep(cx)] + ly[rep(cy)];
+      double* col = piv.data() + (long)cy * c->cxn + cx;
+      for (int k = 0; k < nz; ++k) col[k * P] = b[k] + lxy;
+      for (int k = 1; k < nz; ++k) {
+        const double a_tmp = a[k] / col[(k - 1) * P];
+        col[k * P] -= (a_tmp * cc[k - 1]);
+      }
+    }
+  if (c->cfg.pin_singular) piv[(size_t)(nz - 1) * P] = 0.0;
+
+  PLL_HIP_TRY(hipMalloc(&c->d_a, nz * sizeof(double)));
+  PLL_HIP_TRY(hipMalloc(&c->d_c, nz * sizeof(double)));
+  PLL_HIP_TRY(hipMalloc(&c->d_piv, piv.size() * sizeof(double)));
+  PLL_HIP_TRY(hipMemcpy(c->d_a, a.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+  PLL_HIP_TRY(hipMemcpy(c->d_c, cc.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+  PLL_HIP_TRY(hipMemcpy(c->d_piv, piv.data(), piv.size() * sizeof(double), hipMemcpyHostToDevice));
+  return PLL_OK;
+}
+
+int launch_trid(pll_context* c) {
+  const dim3 grid((c->g.nx + kTx - 1) / kTx, (c->g.ny + kTy - 1) / kTy);
+  TimerScope ts(c, K_TRID);
+  if (c->prec == PLL_FP64)
+    hipLaunchKernelGGL((k_trid<double>), grid, dim3(kTx, kTy), 0, c->stream, c->g.nx, c->g.ny,
+                       c->g.nz, static_cast<double*>(c->spec), c->d_a, c->d_c, c->d_piv, c->cxn,
+                       c->cyn);
+  else
+    hipLaunchKernelGGL((k_trid<float>), grid, dim3(kTx, kTy), 0, c->stream, c->g.nx, c->g.ny,
+                       c->g.nz, static_cast<float*>(c->spec), c->d_a, c->d_c, c->d_piv, c->cxn,
+                       c->cyn);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+}  // namespace pll

@@ -1,0 +1,183 @@
+// pll_internal.hpp -- shared definitions of the HIP library behind include/pll.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pll.h"
+
+namespace pll {
+
+// Halo-padded field geometry: (nx+2)*(ny+2)*(nz+2), x fastest, logical 0..n+1.
+struct Geom {
+  int nx, ny, nz;
+  long sx, sxy;
+  __host__ __device__ long idx(int i, int j, int k) const { return i + sx * j + sxy * k; }
+};
+
+// Mesh coefficient arrays (length nz+2, index 0..nz+1) in the field precision.
+template <typename T>
+struct MeshT {
+  const T* dzf;
+  const T* dzf_inv;
+  const T* dzc_inv;
+  const T* dzflzi;
+};
+
+// Stockham plan of a complex transform of length L held in LDS.
+constexpr int kMaxPass = 20;
+struct FftPlan {
+  int L;
+  int npass;
+  int radix[kMaxPass];
+  int ns[kMaxPass];     // span before the pass (product of earlier radices)
+  int twoff[kMaxPass];  // offset of the pass' twiddles: tw[off + k*(R-1) + r-1]
+};
+
+// Kernel identifiers for the per-kernel timers.
+enum KernelId {
+  K_RK1 = 0,
+  K_RK2,
+  K_RHS,
+  K_FFT_X_FWD,
+  K_FFT_Y_FWD,
+  K_TRID,
+  K_FFT_Y_BWD,
+  K_FFT_X_BWD,
+  K_CORRECT,
+  K_MEAN_PARTIAL,
+  K_MEAN_FINAL,
+  K_FORCE,
+  K_BC_VEL,
+  K_BC_P,
+  K_CHECK,
+  K_INIT,
+  K_MISC,
+  K_COUNT
+};
+extern const char* const kKernelNames[K_COUNT];
+
+// Pending event pair of one timed launch.
+struct PendingTimer {
+  int kid;
+  hipEvent_t start, stop;
+};
+
+}  // namespace pll
+
+struct pll_context {
+  pll_config cfg;
+  pll::Geom g;
+  long nfield;  // halo-padded element count
+  long nspec;   // nx*ny*nz
+  int prec;     // pll_precision
+  size_t esz;   // field element size in bytes
+  hipStream_t stream = nullptr;
+
+  void* fld[PLL_NFIELD] = {};
+  void* spec = nullptr;
+
+  // host mesh in FP64 (index 0..nz+1)
+  std::vector<double> dzf_global, dzf, dzf_inv, dzc, dzc_inv, dzflzi, zc_global;
+  double dx, dy, dx_inv, dy_inv, re_inv, dt, u_crf;
+  double vel_force[3];
+  int is_forced[3];
+
+  void* d_mesh = nullptr;      // 4*(nz+2) in field precision: dzf, dzf_inv, dzc_inv, dzflzi
+  double* d_mesh64 = nullptr;  // same, FP64
+
+  // tridiagonal solve: a, c (nz), pre-factored pivot table [k][cy][cx]
+  int cxn = 0, cyn = 0;  // nx/2+1, ny/2+1
+  double* d_a = nullptr;
+  double* d_c = nullptr;
+  double* d_piv = nullptr;   // FP64 pivots, cxn*cyn*nz
+  float* d_piv32 = nullptr;  // FP32 pivots (FP32-mixed only)
+  float* d_ac32 = nullptr;   // FP32 a, c (FP32-mixed only)
+  double* d_lowk = nullptr;  // FP64 flag table per (cx,cy): 1 = solve in FP64 (FP32-mixed)
+  int nlowk = 0;
+
+  // FFT plans and twiddles (in field precision)
+  pll::FftPlan plan_x;  // complex length nx/2
+  pll::FftPlan plan_y;  // complex length ny
+  void* d_tw_x = nullptr;
+  void* d_tw_y = nullptr;
+  void* d_rtw_x = nullptr;  // exp(-2 pi i k/nx), k = 0..nx/2
+  int x_batch = 1, y_tile = 2;
+
+  // reductions
+  double* d_partial = nullptr;
+  long npartial = 0;
+  int mean_chunks = 1;
+  double* d_scalar = nullptr;  // [0..2] means, [3..5] forces, [6..] scratch
+  void* d_red = nullptr;       // check-reduction scratch
+  long nred = 0;
+
+  // timers
+  bool timing = false;
+  std::vector<pll::PendingTimer> pending;
+  std::vector<hipEvent_t> event_pool;
+  double timer_ms[pll::K_COUNT] = {};
+  long timer_launches[pll::K_COUNT] = {};
+};
+
+namespace pll {
+
+// ---- error reporting --------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+const char* last_error_cstr();
+
+#define PLL_HIP_TRY(expr)                                  \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return ::pll::hip_fail(_e, #expr); \
+  } while (0)
+
+// ---- timers -------------------------------------------------------------------
+void timer_begin(pll_context* c, int kid, PendingTimer* out);
+void timer_end(pll_context* c, PendingTimer* t);
+int timer_collect(pll_context* c);
+
+struct TimerScope {
+  pll_context* c;
+  PendingTimer t;
+  bool on;
+  TimerScope(pll_context* ctx, int kid) : c(ctx), on(ctx->timing) {
+    if (on) timer_begin(c, kid, &t);
+  }
+  ~TimerScope() {
+    if (on) timer_end(c, &t);
+  }
+};
+
+// ---- launchers (implemented in the .hip files) -------------------------------------
+// stencils and pointwise kernels (k_stencil.hip)
+int launch_rk(pll_context* c, int stage);
+int launch_calc_rhs(pll_context* c);
+int launch_correct_vel(pll_context* c);
+int launch_mean(pll_context* c, int field, double nxg, double nyg, double* d_out);
+int launch_force(pll_context* c);
+int launch_bc_vel(pll_context* c, int which);
+int launch_bc_p(pll_context* c);
+int launch_add_scalar(pll_context* c, int field, double value);
+int launch_init_flow(pll_context* c, const double* d_prof, int with_noise, double noise,
+                     const uint32_t seeds[3]);
+int launch_scale_by_inverse_mean(pll_context* c, int field, const double* d_mean, double u_ref);
+int launch_max_div(pll_context* c, pll_value_index* out);
+int launch_max_cfl(pll_context* c, pll_value_index* out);
+int launch_check_nan(pll_context* c, int field, int* passed);
+
+// transforms and the tridiagonal solve (k_fft.hip, k_trid.hip)
+int fft_setup(pll_context* c);
+int launch_fft_x_fwd(pll_context* c);
+int launch_fft_x_bwd(pll_context* c);
+int launch_fft_y_fwd(pll_context* c);
+int launch_fft_y_bwd(pll_context* c);
+int trid_setup(pll_context* c);
+int launch_trid(pll_context* c);
+
+}  // namespace pll
