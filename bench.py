@@ -55,7 +55,30 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time a plain device copy of 1 GiB (bandwidth ceiling and the "
+                         "scale factor of the FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe():
+    """Plain device-to-device copy of 2^27 doubles (1 GiB read + 1 GiB written)."""
+    import torch
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    gbs = 2 * n * 8 / sec / 1e9
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
+                      "copy_GBps": round(gbs, 1), "copy_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}))
 
 
 def build_channel(wl, precision):
@@ -94,6 +117,9 @@ def cpu_baseline(wl, steps):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(the multi-GPU sharding is a design, see DESIGN.md)"}))

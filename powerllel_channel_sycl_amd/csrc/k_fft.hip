@@ -17,83 +17,16 @@
 // Complex transforms use the Stockham autosort formulation with radix 4, 2,
 // 3 and 5 passes ping-ponging between two LDS buffers.  Twiddles are computed
 // on the host in long double and rounded to the field precision.
+#include "fft_radix.hpp"
 #include "pll_internal.hpp"
 
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 
 namespace pll {
 
 namespace {
-
-template <typename T>
-struct cpx {
-  T re, im;
-};
-
-template <typename T>
-__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
-  return cpx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-
-template <typename T>
-__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
-  return cpx<T>{a.re + b.re, a.im + b.im};
-}
-
-template <typename T>
-__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
-  return cpx<T>{a.re - b.re, a.im - b.im};
-}
-
-// s * i * a for s = +1 / -1
-template <typename T, int S>
-__device__ __forceinline__ cpx<T> rot(cpx<T> a) {
-  return S > 0 ? cpx<T>{-a.im, a.re} : cpx<T>{a.im, -a.re};
-}
-
-// In-register DFT of R points, X_q = sum_r v_r exp(S*2*pi*i*r*q/R), S = -1
-// forward, +1 inverse.  Constants are the exact trigonometric values.
-template <typename T, int R, int S>
-__device__ __forceinline__ void dft(cpx<T>* v) {
-  if (R == 2) {
-    const cpx<T> a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-  } else if (R == 4) {
-    const cpx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    const cpx<T> s13 = cadd(v[1], v[3]), d13 = rot<T, S>(csub(v[1], v[3]));
-    v[0] = cadd(s02, s13);
-    v[2] = csub(s02, s13);
-    v[1] = cadd(d02, d13);
-    v[3] = csub(d02, d13);
-  } else if (R == 3) {
-    const T h3 = T(0.86602540378443864676372317075294);  // sin(pi/3)
-    const cpx<T> s = cadd(v[1], v[2]);
-    const cpx<T> d = csub(v[1], v[2]);
-    const cpx<T> m{v[0].re - T(0.5) * s.re, v[0].im - T(0.5) * s.im};
-    const cpx<T> r = rot<T, S>(cpx<T>{h3 * d.re, h3 * d.im});
-    v[0] = cadd(v[0], s);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
-  } else if (R == 5) {
-    const T c1 = T(0.30901699437494742410229341718282);   // cos(2pi/5)
-    const T c2 = T(-0.80901699437494742410229341718282);  // cos(4pi/5)
-    const T s1 = T(0.95105651629515357211643933337938);   // sin(2pi/5)
-    const T s2 = T(0.58778525229247312916870595463907);   // sin(4pi/5)
-    const cpx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
-    const cpx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
-    const cpx<T> m1{v[0].re + c1 * a1.re + c2 * a2.re, v[0].im + c1 * a1.im + c2 * a2.im};
-    const cpx<T> m2{v[0].re + c2 * a1.re + c1 * a2.re, v[0].im + c2 * a1.im + c1 * a2.im};
-    const cpx<T> n1 = rot<T, S>(cpx<T>{s1 * b1.re + s2 * b2.re, s1 * b1.im + s2 * b2.im});
-    const cpx<T> n2 = rot<T, S>(cpx<T>{s2 * b1.re - s1 * b2.re, s2 * b1.im - s1 * b2.im});
-    v[0] = cadd(v[0], cadd(a1, a2));
-    v[1] = cadd(m1, n1);
-    v[4] = csub(m1, n1);
-    v[2] = cadd(m2, n2);
-    v[3] = csub(m2, n2);
-  }
-}
 
 // One Stockham butterfly: sequence q, butterfly jj of a radix-R pass with span Ns.
 template <typename T, int R, int S>
@@ -347,6 +280,303 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y_bwd(Geom g, T* __restrict
   }
 }
 
+// ============================================================================
+// Compile-time transforms for the lengths of the supported grids.  One
+// padded LDS buffer per block; each pass reads its butterfly inputs into
+// registers, synchronises, and writes the outputs back in place, so every
+// thread does at most one radix-R butterfly per pass.  Index arithmetic is
+// constant-folded and the 6/8/12/16 butterflies keep the pass count at 2-3.
+// ============================================================================
+template <int L>
+struct Shape {
+  static constexpr int np = 0;
+  static constexpr int r[1] = {0};
+};
+#define PLL_SHAPE(LEN, ...)                          \
+  template <>                                        \
+  struct Shape<LEN> {                                \
+    static constexpr int r[] = {__VA_ARGS__};        \
+    static constexpr int np = sizeof(r) / sizeof(int); \
+  };
+PLL_SHAPE(4, 4)
+PLL_SHAPE(6, 6)
+PLL_SHAPE(12, 12)
+PLL_SHAPE(20, 4, 5)
+PLL_SHAPE(32, 8, 4)
+PLL_SHAPE(48, 8, 6)
+PLL_SHAPE(64, 8, 8)
+PLL_SHAPE(128, 16, 8)
+PLL_SHAPE(192, 16, 12)
+PLL_SHAPE(256, 16, 16)
+PLL_SHAPE(384, 8, 8, 6)
+PLL_SHAPE(768, 8, 8, 12)
+#undef PLL_SHAPE
+
+template <class SH>
+constexpr int shape_rmin() {
+  int m = 1 << 30;
+  for (int p = 0; p < SH::np; ++p) m = SH::r[p] < m ? SH::r[p] : m;
+  return m;
+}
+template <class SH, int P>
+constexpr int shape_span() {
+  int s = 1;
+  for (int p = 0; p < P; ++p) s *= SH::r[p];
+  return s;
+}
+// threads per sequence: every thread owns one butterfly of the smallest radix
+template <int L>
+constexpr int tps() {
+  return L / shape_rmin<Shape<L>>();
+}
+// one pad slot per 16 complex values spreads strided accesses over the banks
+__device__ __forceinline__ int pad(int x) { return x + (x >> 4); }
+constexpr int padded(int n) { return n + (n >> 4) + 1; }
+constexpr int kLdsMax = 64 * 1024;
+
+// rows per block of the x transforms (about 256 threads, LDS <= 64 KiB)
+template <typename T, int L>
+constexpr int x_rows() {
+  int nb = tps<L>() >= 256 ? 1 : 256 / tps<L>();
+  while (nb > 1 && padded(nb * L) * 2 * (int)sizeof(T) > kLdsMax) nb /= 2;
+  return nb;
+}
+// sequences per block of the y transforms: 128-byte rows, at most 1024 threads
+// and 64 KiB of LDS
+template <typename T, int L>
+constexpr int y_seqs() {
+  int q = 128 / (2 * (int)sizeof(T));
+  while (q > 1 && (q * tps<L>() > 1024 || padded(L * q) * 2 * (int)sizeof(T) > kLdsMax)) q /= 2;
+  return q;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical blocks so neighbouring tiles share its L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
+  const unsigned q = n / 8, r = n % 8, x = b % 8, l = b / 8;
+  return x * q + (x < r ? x : r) + l;
+}
+
+template <typename T, int S, int L, int R, int NS, int ES>
+__device__ __forceinline__ void stockham_pass(cpx<T>* buf, const cpx<T>* __restrict__ twL,
+                                              int soff, int jj) {
+  constexpr int LR = L / R;
+  cpx<T> v[R];
+  const bool active = jj < LR;
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = buf[pad(soff + (jj + r * LR) * ES)];
+  }
+  __syncthreads();
+  if (active) {
+    const int k = jj % NS;
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<T> w = twL[r * k * (L / (NS * R))];
+        if (S > 0) w.im = -w.im;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    dft<T, R, S>(v);
+    const int base = (jj / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[pad(soff + (base + r * NS) * ES)] = v[r];
+  }
+  __syncthreads();
+}
+
+template <typename T, int S, int L, int ES, int P = 0>
+__device__ __forceinline__ void stockham(cpx<T>* buf, const cpx<T>* __restrict__ twL, int soff,
+                                         int jj) {
+  using SH = Shape<L>;
+  if constexpr (P < SH::np) {
+    stockham_pass<T, S, L, SH::r[P], shape_span<SH, P>(), ES>(buf, twL, soff, jj);
+    stockham<T, S, L, ES, P + 1>(buf, twL, soff, jj);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T& real_at(cpx<T>* buf, int e) {
+  return reinterpret_cast<T*>(buf)[2 * pad(e >> 1) + (e & 1)];
+}
+
+// execute_fft_fwd_xpen (src/mod_fft.cc:60-88), L = nx/2 points per row.
+template <typename T, int L>
+__global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xf(Geom g, const T* __restrict__ p,
+                                                               T* __restrict__ spec,
+                                                               const cpx<T>* __restrict__ twL,
+                                                               const cpx<T>* __restrict__ rtw,
+                                                               long nrows) {
+  constexpr int TPS = tps<L>(), NB = x_rows<T, L>(), NT = NB * TPS, N = 2 * L;
+  __shared__ cpx<T> buf[padded(NB * L)];
+  __shared__ long rowbase[NB];
+  const long row0 = (long)xcd_remap(blockIdx.x, gridDim.x) * NB;
+  if (threadIdx.x < NB) {
+    const long r = row0 + threadIdx.x;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    rowbase[threadIdx.x] = r < nrows ? g.idx(1, j + 1, k + 1) : -1;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NB * N; e += NT) {
+    const int s = e / N, i = e % N;
+    const long b = rowbase[s];
+    real_at(buf, s * L * 2 + i) = b >= 0 ? p[b + i] : T(0);
+  }
+  __syncthreads();
+  const int s = threadIdx.x / TPS, jj = threadIdx.x % TPS;
+  stockham<T, -1, L, 1>(buf, twL, s * L, jj);
+  const long r = row0 + s;
+  if (r >= nrows) return;
+  cpx<T>* out = reinterpret_cast<cpx<T>*>(spec + r * N);
+  for (int kk = jj; kk < L; kk += TPS) {
+    const cpx<T> zk = buf[pad(s * L + kk)];
+    if (kk == 0) {
+      out[0] = cpx<T>{zk.re + zk.im, zk.re - zk.im};  // X0, X(n/2)
+    } else {
+      const cpx<T> zm = buf[pad(s * L + (L - kk))];
+      const T er = T(0.5) * (zk.re + zm.re), ei = T(0.5) * (zk.im - zm.im);
+      const T orr = T(0.5) * (zk.im + zm.im), oi = T(-0.5) * (zk.re - zm.re);
+      const cpx<T> w = rtw[kk];
+      out[kk] = cpx<T>{er + (w.re * orr - w.im * oi), ei + (w.re * oi + w.im * orr)};
+    }
+  }
+}
+
+// execute_fft_bwd_xpen (src/mod_fft.cc:90-120): perm rows -> p interior * norm.
+template <typename T, int L>
+__global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xb(Geom g, const T* __restrict__ spec,
+                                                               T* __restrict__ p,
+                                                               const cpx<T>* __restrict__ twL,
+                                                               const cpx<T>* __restrict__ rtw,
+                                                               long nrows, T norm) {
+  constexpr int TPS = tps<L>(), NB = x_rows<T, L>(), NT = NB * TPS, N = 2 * L;
+  constexpr int RMIN = shape_rmin<Shape<L>>();
+  __shared__ cpx<T> buf[padded(NB * L)];
+  __shared__ long rowbase[NB];
+  const long row0 = (long)xcd_remap(blockIdx.x, gridDim.x) * NB;
+  if (threadIdx.x < NB) {
+    const long r = row0 + threadIdx.x;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+    rowbase[threadIdx.x] = r < nrows ? g.idx(1, j + 1, k + 1) : -1;
+  }
+  for (int e = threadIdx.x; e < NB * N; e += NT) {
+    const int s = e / N, i = e % N;
+    const long r = row0 + s;
+    real_at(buf, s * N + i) = r < nrows ? spec[r * N + i] : T(0);
+  }
+  __syncthreads();
+  const int s = threadIdx.x / TPS, jj = threadIdx.x % TPS;
+  // perm slot kk holds (Re Xkk, Im Xkk); slot 0 holds (X0, X(n/2))
+  cpx<T> z[RMIN];
+#pragma unroll
+  for (int u = 0; u < RMIN; ++u) {
+    const int kk = jj + u * TPS;
+    const cpx<T> a = buf[pad(s * L + kk)];
+    cpx<T> xk, xm;
+    if (kk == 0) {
+      xk = cpx<T>{a.re, T(0)};
+      xm = cpx<T>{a.im, T(0)};
+    } else {
+      xk = a;
+      xm = buf[pad(s * L + (L - kk))];
+    }
+    const cpx<T> sm{xk.re + xm.re, xk.im - xm.im};  // Xk + conj(X(h-k))
+    const cpx<T> d{xk.re - xm.re, xk.im + xm.im};   // Xk - conj(X(h-k))
+    const cpx<T> w{rtw[kk].re, -rtw[kk].im};        // exp(+2 pi i k/n)
+    const cpx<T> wd = cmul(w, d);
+    z[u] = cpx<T>{sm.re - wd.im, sm.im + wd.re};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < RMIN; ++u) buf[pad(s * L + jj + u * TPS)] = z[u];
+  __syncthreads();
+  stockham<T, +1, L, 1>(buf, twL, s * L, jj);
+  for (int e = threadIdx.x; e < NB * N; e += NT) {
+    const int s2 = e / N, i = e % N;
+    const long b = rowbase[s2];
+    if (b >= 0) p[b + i] = norm * real_at(buf, s2 * N + i);
+  }
+}
+
+// execute_fft_fwd_ypen (src/mod_fft.cc:122-153) on a tile of 2Q perm columns.
+template <typename T, int L>
+__global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yf(Geom g, T* __restrict__ spec,
+                                                                  const cpx<T>* __restrict__ twL,
+                                                                  int ntiles) {
+  constexpr int TPS = tps<L>(), Q = y_seqs<T, L>(), NT = Q * TPS, W = 2 * Q;
+  __shared__ cpx<T> buf[padded(L * Q)];
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = (int)(b % (unsigned)ntiles), k = (int)(b / (unsigned)ntiles);
+  T* plane = spec + (long)k * g.nx * g.ny + (long)tile * W;
+  for (int e = threadIdx.x; e < L * W; e += NT) {
+    const int j = e / W, c = e % W;
+    real_at(buf, j * W + c) = plane[(long)j * g.nx + c];
+  }
+  __syncthreads();
+  const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
+  stockham<T, -1, L, Q>(buf, twL, q, jj);
+  for (int m = jj; m <= L / 2; m += TPS) {
+    const cpx<T> zm = buf[pad(m * Q + q)];
+    const cpx<T> zc = buf[pad(((L - m) % L) * Q + q)];
+    const T ar = T(0.5) * (zm.re + zc.re), ai = T(0.5) * (zm.im - zc.im);
+    const T br = T(0.5) * (zm.im + zc.im), bi = T(-0.5) * (zm.re - zc.re);
+    if (m == 0) {
+      reinterpret_cast<cpx<T>*>(plane)[q] = cpx<T>{ar, br};
+    } else if (m == L / 2) {
+      reinterpret_cast<cpx<T>*>(plane + g.nx)[q] = cpx<T>{ar, br};
+    } else {
+      reinterpret_cast<cpx<T>*>(plane + (long)(2 * m) * g.nx)[q] = cpx<T>{ar, br};
+      reinterpret_cast<cpx<T>*>(plane + (long)(2 * m + 1) * g.nx)[q] = cpx<T>{ai, bi};
+    }
+  }
+}
+
+// execute_fft_bwd_ypen (src/mod_fft.cc:155-187), unscaled.
+template <typename T, int L>
+__global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yb(Geom g, T* __restrict__ spec,
+                                                                  const cpx<T>* __restrict__ twL,
+                                                                  int ntiles) {
+  constexpr int TPS = tps<L>(), Q = y_seqs<T, L>(), NT = Q * TPS, W = 2 * Q;
+  constexpr int RMIN = shape_rmin<Shape<L>>();
+  __shared__ cpx<T> buf[padded(L * Q)];
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = (int)(b % (unsigned)ntiles), k = (int)(b / (unsigned)ntiles);
+  T* plane = spec + (long)k * g.nx * g.ny + (long)tile * W;
+  for (int e = threadIdx.x; e < L * W; e += NT) {
+    const int j = e / W, c = e % W;
+    real_at(buf, j * W + c) = plane[(long)j * g.nx + c];
+  }
+  __syncthreads();
+  const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
+  // slot (j, q) holds (perm_a[j], perm_b[j]) of columns 2q and 2q+1
+  cpx<T> z[RMIN];
+#pragma unroll
+  for (int u = 0; u < RMIN; ++u) {
+    const int m = jj + u * TPS;
+    if (m == 0) {
+      z[u] = buf[pad(q)];
+    } else if (m == L / 2) {
+      z[u] = buf[pad(Q + q)];
+    } else {
+      const int mm = m < L / 2 ? m : L - m;
+      const cpx<T> e0 = buf[pad(2 * mm * Q + q)], e1 = buf[pad((2 * mm + 1) * Q + q)];
+      // A = (e0.re, e1.re), B = (e0.im, e1.im); Z = A + iB, or conj(A) + i conj(B) past L/2
+      z[u] = m < L / 2 ? cpx<T>{e0.re - e1.im, e1.re + e0.im}
+                       : cpx<T>{e0.re + e1.im, e0.im - e1.re};
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < RMIN; ++u) buf[pad((jj + u * TPS) * Q + q)] = z[u];
+  __syncthreads();
+  stockham<T, +1, L, Q>(buf, twL, q, jj);
+  for (int e = threadIdx.x; e < L * W; e += NT) {
+    const int j = e / W, c = e % W;
+    plane[(long)j * g.nx + c] = real_at(buf, j * W + c);
+  }
+}
+
 // ---- host side -------------------------------------------------------------------
 bool build_plan(int L, FftPlan& pl, std::vector<std::complex<long double>>& tw) {
   pl.L = L;
@@ -418,7 +648,7 @@ int setup_t(pll_context* c) {
 }
 
 template <typename T>
-int x_fwd_t(pll_context* c) {
+int x_fwd_generic(pll_context* c) {
   const long nrows = (long)c->g.ny * c->g.nz;
   const unsigned blocks = (unsigned)((nrows + c->x_batch - 1) / c->x_batch);
   const size_t sh = (size_t)c->x_batch * c->g.nx * sizeof(T) * 2;
@@ -432,7 +662,7 @@ int x_fwd_t(pll_context* c) {
 }
 
 template <typename T>
-int x_bwd_t(pll_context* c) {
+int x_bwd_generic(pll_context* c) {
   const long nrows = (long)c->g.ny * c->g.nz;
   const unsigned blocks = (unsigned)((nrows + c->x_batch - 1) / c->x_batch);
   const size_t sh = (size_t)c->x_batch * c->g.nx * sizeof(T) * 2;
@@ -447,7 +677,7 @@ int x_bwd_t(pll_context* c) {
 }
 
 template <typename T>
-int y_t(pll_context* c, bool fwd) {
+int y_generic(pll_context* c, bool fwd) {
   const dim3 grid(c->g.nx / c->y_tile, c->g.nz);
   const size_t sh = (size_t)c->y_tile * c->g.ny * sizeof(T) * 2;
   if (fwd) {
@@ -465,11 +695,132 @@ int y_t(pll_context* c, bool fwd) {
   return PLL_OK;
 }
 
+// ---- fixed-length kernels ------------------------------------------------------------
+template <typename T, int L>
+int x_fixed(pll_context* c, bool fwd) {
+  constexpr int NB = x_rows<T, L>(), NT = NB * tps<L>();
+  const long nrows = (long)c->g.ny * c->g.nz;
+  const unsigned blocks = (unsigned)((nrows + NB - 1) / NB);
+  const cpx<T>* tw = static_cast<const cpx<T>*>(c->d_twl_x);
+  const cpx<T>* rtw = static_cast<const cpx<T>*>(c->d_rtw_x);
+  if (fwd) {
+    TimerScope ts(c, K_FFT_X_FWD);
+    hipLaunchKernelGGL((k_xf<T, L>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+                       static_cast<const T*>(c->fld[PLL_P]), static_cast<T*>(c->spec), tw, rtw,
+                       nrows);
+  } else {
+    const double norm = 1.0 / (c->g.nx * c->g.ny);  // src/mod_fft.cc:57
+    TimerScope ts(c, K_FFT_X_BWD);
+    hipLaunchKernelGGL((k_xb<T, L>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+                       static_cast<const T*>(c->spec), static_cast<T*>(c->fld[PLL_P]), tw, rtw,
+                       nrows, (T)norm);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename T, int L>
+bool y_fixed_fits(const pll_context* c) {
+  return c->g.nx % (2 * y_seqs<T, L>()) == 0;
+}
+
+template <typename T, int L>
+int y_fixed(pll_context* c, bool fwd) {
+  constexpr int Q = y_seqs<T, L>(), NT = Q * tps<L>();
+  const int ntiles = c->g.nx / (2 * Q);
+  const unsigned blocks = (unsigned)ntiles * c->g.nz;
+  const cpx<T>* tw = static_cast<const cpx<T>*>(c->d_twl_y);
+  if (fwd) {
+    TimerScope ts(c, K_FFT_Y_FWD);
+    hipLaunchKernelGGL((k_yf<T, L>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+                       static_cast<T*>(c->spec), tw, ntiles);
+  } else {
+    TimerScope ts(c, K_FFT_Y_BWD);
+    hipLaunchKernelGGL((k_yb<T, L>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+                       static_cast<T*>(c->spec), tw, ntiles);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+#define PLL_X_SIZES(X) X(4) X(12) X(32) X(48) X(128) X(192) X(384) X(768)
+#define PLL_Y_SIZES(X) X(6) X(20) X(48) X(64) X(192) X(256) X(384) X(768)
+
+template <typename T>
+bool x_has_fixed(int L) {
+#define PLL_CASE(n) if (L == n) return true;
+  PLL_X_SIZES(PLL_CASE)
+#undef PLL_CASE
+  return false;
+}
+
+template <typename T>
+bool y_has_fixed(const pll_context* c, int L) {
+#define PLL_CASE(n) if (L == n) return y_fixed_fits<T, n>(c);
+  PLL_Y_SIZES(PLL_CASE)
+#undef PLL_CASE
+  return false;
+}
+
+template <typename T>
+int x_t(pll_context* c, bool fwd) {
+  if (c->x_fixed) {
+    switch (c->g.nx / 2) {
+#define PLL_CASE(n) case n: return x_fixed<T, n>(c, fwd);
+      PLL_X_SIZES(PLL_CASE)
+#undef PLL_CASE
+      default: break;
+    }
+  }
+  return fwd ? x_fwd_generic<T>(c) : x_bwd_generic<T>(c);
+}
+
+template <typename T>
+int y_t(pll_context* c, bool fwd) {
+  if (c->y_fixed) {
+    switch (c->g.ny) {
+#define PLL_CASE(n) case n: return y_fixed<T, n>(c, fwd);
+      PLL_Y_SIZES(PLL_CASE)
+#undef PLL_CASE
+      default: break;
+    }
+  }
+  return y_generic<T>(c, fwd);
+}
+
+// W_L^t = exp(-2 pi i t / L), t = 0..L-1, for the fixed-length kernels.
+template <typename T>
+int unit_table(int L, void** dst) {
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  std::vector<std::complex<long double>> t;
+  for (int k = 0; k < L; ++k) {
+    const long double ang = -two_pi * (long double)k / (long double)L;
+    t.emplace_back(cosl(ang), sinl(ang));
+  }
+  return upload_table<T>(t, dst);
+}
+
+template <typename T>
+int setup_fixed(pll_context* c) {
+  const char* env = std::getenv("PLL_FFT_GENERIC");
+  const bool force_generic = env && env[0] == '1';
+  c->x_fixed = !force_generic && x_has_fixed<T>(c->g.nx / 2);
+  c->y_fixed = !force_generic && y_has_fixed<T>(c, c->g.ny);
+  int rc;
+  if ((rc = unit_table<T>(c->g.nx / 2, &c->d_twl_x))) return rc;
+  if ((rc = unit_table<T>(c->g.ny, &c->d_twl_y))) return rc;
+  return PLL_OK;
+}
+
 }  // namespace
 
-int fft_setup(pll_context* c) { return c->prec == PLL_FP64 ? setup_t<double>(c) : setup_t<float>(c); }
-int launch_fft_x_fwd(pll_context* c) { return c->prec == PLL_FP64 ? x_fwd_t<double>(c) : x_fwd_t<float>(c); }
-int launch_fft_x_bwd(pll_context* c) { return c->prec == PLL_FP64 ? x_bwd_t<double>(c) : x_bwd_t<float>(c); }
+int fft_setup(pll_context* c) {
+  int rc = c->prec == PLL_FP64 ? setup_t<double>(c) : setup_t<float>(c);
+  if (rc) return rc;
+  return c->prec == PLL_FP64 ? setup_fixed<double>(c) : setup_fixed<float>(c);
+}
+int launch_fft_x_fwd(pll_context* c) { return c->prec == PLL_FP64 ? x_t<double>(c, true) : x_t<float>(c, true); }
+int launch_fft_x_bwd(pll_context* c) { return c->prec == PLL_FP64 ? x_t<double>(c, false) : x_t<float>(c, false); }
 int launch_fft_y_fwd(pll_context* c) { return c->prec == PLL_FP64 ? y_t<double>(c, true) : y_t<float>(c, true); }
 int launch_fft_y_bwd(pll_context* c) { return c->prec == PLL_FP64 ? y_t<double>(c, false) : y_t<float>(c, false); }
 

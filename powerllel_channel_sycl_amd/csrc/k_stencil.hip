@@ -24,11 +24,10 @@ constexpr int kRedThreads = 256;
 // (i,j,k), src/mod_calcVel.cc:61-125 (stage 1) and :174-238 (stage 2).  Faces:
 // u at (i+1/2, j, k), v at (i, j+1/2, k), w at (i, j, k+1/2).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void tendency(const T* __restrict__ u, const T* __restrict__ v,
-                                         const T* __restrict__ w, const MeshT<T>& m, long c,
-                                         long ys, long zs, int k, T dxi, T dyi, T rei, T ucrf,
-                                         T& tu, T& tv, T& tw) {
+template <typename T, class A>
+__device__ __forceinline__ void tendency(const A& U, const A& V, const A& W, const MeshT<T>& m,
+                                         int k, T dxi, T dyi, T rei, T ucrf, T& tu, T& tv,
+                                         T& tw) {
   const T quarter = T(0.25), half = T(0.5);
   const T dzf_k = m.dzf[k], dzf_p = m.dzf[k + 1], dzf_m = m.dzf[k - 1];
   const T dci_k = m.dzc_inv[k], dci_m = m.dzc_inv[k - 1];
@@ -36,15 +35,15 @@ __device__ __forceinline__ void tendency(const T* __restrict__ u, const T* __res
 
   // ---- x-momentum at the u face ----
   {
-    const T q0 = u[c], qe = u[c + 1], qw = u[c - 1], qn = u[c + ys], qs = u[c - ys];
-    const T qt = u[c + zs], qb = u[c - zs];
+    const T q0 = U(0, 0, 0), qe = U(1, 0, 0), qw = U(-1, 0, 0), qn = U(0, 1, 0);
+    const T qs = U(0, -1, 0), qt = U(0, 0, 1), qb = U(0, 0, -1);
     const T ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
     const T at = (q0 * dzf_p + qt * dzf_k) * dci_k;
     const T ab = (q0 * dzf_m + qb * dzf_k) * dci_m;
-    const T vn = v[c] + v[c + 1];
-    const T vs = v[c - ys] + v[c - ys + 1];
-    const T wt = w[c] + w[c + 1];
-    const T wb = w[c - zs] + w[c - zs + 1];
+    const T vn = V(0, 0, 0) + V(1, 0, 0);
+    const T vs = V(0, -1, 0) + V(1, -1, 0);
+    const T wt = W(0, 0, 0) + W(1, 0, 0);
+    const T wb = W(0, 0, -1) + W(1, 0, -1);
     const T ge = (qe - q0) * dxi, gw = (q0 - qw) * dxi;
     const T gn = (qn - q0) * dyi, gs = (q0 - qs) * dyi;
     const T gt = (qt - q0) * dci_k, gb = (q0 - qb) * dci_m;
@@ -56,15 +55,15 @@ __device__ __forceinline__ void tendency(const T* __restrict__ u, const T* __res
   }
   // ---- y-momentum at the v face ----
   {
-    const T q0 = v[c], qe = v[c + 1], qw = v[c - 1], qn = v[c + ys], qs = v[c - ys];
-    const T qt = v[c + zs], qb = v[c - zs];
+    const T q0 = V(0, 0, 0), qe = V(1, 0, 0), qw = V(-1, 0, 0), qn = V(0, 1, 0);
+    const T qs = V(0, -1, 0), qt = V(0, 0, 1), qb = V(0, 0, -1);
     const T ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
     const T at = (q0 * dzf_p + qt * dzf_k) * dci_k;
     const T ab = (q0 * dzf_m + qb * dzf_k) * dci_m;
-    const T ue = u[c] + u[c + ys];
-    const T uw = u[c - 1] + u[c - 1 + ys];
-    const T wt = w[c] + w[c + ys];
-    const T wb = w[c - zs] + w[c - zs + ys];
+    const T ue = U(0, 0, 0) + U(0, 1, 0);
+    const T uw = U(-1, 0, 0) + U(-1, 1, 0);
+    const T wt = W(0, 0, 0) + W(0, 1, 0);
+    const T wb = W(0, 0, -1) + W(0, 1, -1);
     const T ge = (qe - q0) * dxi, gw = (q0 - qw) * dxi;
     const T gn = (qn - q0) * dyi, gs = (q0 - qs) * dyi;
     const T gt = (qt - q0) * dci_k, gb = (q0 - qb) * dci_m;
@@ -76,14 +75,14 @@ __device__ __forceinline__ void tendency(const T* __restrict__ u, const T* __res
   }
   // ---- z-momentum at the w face ----
   {
-    const T q0 = w[c], qe = w[c + 1], qw = w[c - 1], qn = w[c + ys], qs = w[c - ys];
-    const T qt = w[c + zs], qb = w[c - zs];
+    const T q0 = W(0, 0, 0), qe = W(1, 0, 0), qw = W(-1, 0, 0), qn = W(0, 1, 0);
+    const T qs = W(0, -1, 0), qt = W(0, 0, 1), qb = W(0, 0, -1);
     const T ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
     const T at = q0 + qt, ab = q0 + qb;
-    const T ue = (u[c] * dzf_p + u[c + zs] * dzf_k) * dci_k;
-    const T uw = (u[c - 1] * dzf_p + u[c - 1 + zs] * dzf_k) * dci_k;
-    const T vn = (v[c] * dzf_p + v[c + zs] * dzf_k) * dci_k;
-    const T vs = (v[c - ys] * dzf_p + v[c - ys + zs] * dzf_k) * dci_k;
+    const T ue = (U(0, 0, 0) * dzf_p + U(0, 0, 1) * dzf_k) * dci_k;
+    const T uw = (U(-1, 0, 0) * dzf_p + U(-1, 0, 1) * dzf_k) * dci_k;
+    const T vn = (V(0, 0, 0) * dzf_p + V(0, 0, 1) * dzf_k) * dci_k;
+    const T vs = (V(0, -1, 0) * dzf_p + V(0, -1, 1) * dzf_k) * dci_k;
     const T ge = (qe - q0) * dxi, gw = (q0 - qw) * dxi;
     const T gn = (qn - q0) * dyi, gs = (q0 - qs) * dyi;
     const T gt = (qt - q0) * dfi_p, gb = (q0 - qb) * dfi_k;
@@ -95,75 +94,171 @@ __device__ __forceinline__ void tendency(const T* __restrict__ u, const T* __res
   }
 }
 
+// Stencil value (di, dj, dk) around the thread's cell, read from an LDS ring
+// of three tile planes (k-1, k, k+1) of width SX.
+template <typename T, int SX>
+struct PlaneAcc {
+  const T* m;
+  const T* z;
+  const T* p;
+  int c;
+  __device__ __forceinline__ T operator()(int di, int dj, int dk) const {
+    const T* pl = dk < 0 ? m : (dk > 0 ? p : z);
+    return pl[c + di + dj * SX];
+  }
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical blocks so neighbouring tiles share its L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
+  const unsigned q = n / 8, r = n % 8, x = b % 8, l = b / 8;
+  return x * q + (x < r ? x : r) + l;
+}
+
+constexpr int kRkTx = 64;  // tile width along x (one wavefront per row)
+constexpr int kRkTy = 8;   // tile rows along y
+
 // Stage 1: d = s + dt*F(s).  Stage 2: d = (s + dt*F(s) + d)/2.
-// One thread per (i,j) column, marching over a chunk of k planes so the
-// vertical neighbours of consecutive iterations stay in the CU's caches.
+// A block owns a 64x8 (i,j) tile and marches a chunk of k planes.  The three
+// source fields of planes k-1, k, k+1 (tile plus a one-cell halo) sit in an
+// LDS ring; plane k+2 is fetched into registers while plane k is computed and
+// replaces plane k-1 afterwards, so every source value leaves HBM once per block.
 template <typename T, int STAGE>
-__global__ __launch_bounds__(kTx* kTy) void k_rk(Geom g, const T* __restrict__ su,
-                                                 const T* __restrict__ sv,
-                                                 const T* __restrict__ sw, T* __restrict__ du,
-                                                 T* __restrict__ dv, T* __restrict__ dw,
-                                                 MeshT<T> m, T dxi, T dyi, T rei, T dt, T ucrf,
-                                                 int kchunk) {
-  const int i = blockIdx.x * kTx + threadIdx.x + 1;
-  const int j = blockIdx.y * kTy + threadIdx.y + 1;
-  if (i > g.nx || j > g.ny) return;
-  const int k0 = blockIdx.z * kchunk + 1;
-  const int k1 = min(k0 + kchunk - 1, g.nz);
-  const long ys = g.sx, zs = g.sxy;
+__global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict__ su,
+                                                     const T* __restrict__ sv,
+                                                     const T* __restrict__ sw, T* __restrict__ du,
+                                                     T* __restrict__ dv, T* __restrict__ dw,
+                                                     MeshT<T> m, T dxi, T dyi, T rei, T dt,
+                                                     T ucrf, int kchunk, int ntx, int nty) {
+  constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY, NT = TX * TY;
+  constexpr int NPF = (PL + NT - 1) / NT;
+  __shared__ T sh[3][3][PL];  // [field][ring slot][tile cell]
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
+  const int kc = (int)(b / (unsigned)(ntx * nty));
+  const int i0 = ti * TX, j0 = tj * TY;  // tile cell (0,0) is the halo corner (i0, j0)
+  const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int i = i0 + tx + 1, j = j0 + ty + 1;
+  const bool active = i <= g.nx && j <= g.ny;
+
+  long gaddr[NPF];  // plane-relative address of this thread's tile cells (-1: outside)
+  int cell[NPF];
+#pragma unroll
+  for (int n = 0; n < NPF; ++n) {
+    const int e = threadIdx.x + n * NT;
+    const int gx = i0 + e % SX, gy = j0 + e / SX;
+    cell[n] = e;
+    gaddr[n] = (e < PL && gx <= g.nx + 1 && gy <= g.ny + 1) ? gx + g.sx * gy : -1;
+  }
+  T pre[3][NPF];
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        const long a = gaddr[n] + g.sxy * kk;
+        pre[0][n] = su[a];
+        pre[1][n] = sv[a];
+        pre[2][n] = sw[a];
+      }
+  };
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        sh[0][slot][cell[n]] = pre[0][n];
+        sh[1][slot][cell[n]] = pre[1][n];
+        sh[2][slot][cell[n]] = pre[2][n];
+      }
+  };
+  if (k0 > k1) return;
+  fetch(k0 - 1);
+  commit(0);
+  fetch(k0);
+  commit(1);
+  fetch(k0 + 1);
+  commit(2);
+  __syncthreads();
+  int sm = 0, s0 = 1, sp = 2;
+  const int c = (ty + 1) * SX + tx + 1;
   for (int k = k0; k <= k1; ++k) {
-    const long c = g.idx(i, j, k);
-    T tu, tv, tw;
-    tendency<T>(su, sv, sw, m, c, ys, zs, k, dxi, dyi, rei, ucrf, tu, tv, tw);
-    if (STAGE == 1) {
-      du[c] = su[c] + dt * tu;
-      dv[c] = sv[c] + dt * tv;
-      dw[c] = sw[c] + dt * tw;
-    } else {
-      du[c] = (su[c] + dt * tu + du[c]) * T(0.5);
-      dv[c] = (sv[c] + dt * tv + dv[c]) * T(0.5);
-      dw[c] = (sw[c] + dt * tw + dw[c]) * T(0.5);
+    const bool more = k + 2 <= k1 + 1;
+    if (more) fetch(k + 2);
+    if (active) {
+      const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
+      const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
+      const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
+      T tu, tv, tw;
+      tendency<T>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw);
+      const long a = g.idx(i, j, k);
+      if (STAGE == 1) {
+        du[a] = U(0, 0, 0) + dt * tu;
+        dv[a] = V(0, 0, 0) + dt * tv;
+        dw[a] = W(0, 0, 0) + dt * tw;
+      } else {
+        du[a] = (U(0, 0, 0) + dt * tu + du[a]) * T(0.5);
+        dv[a] = (V(0, 0, 0) + dt * tv + dv[a]) * T(0.5);
+        dw[a] = (W(0, 0, 0) + dt * tw + dw[a]) * T(0.5);
+      }
     }
+    __syncthreads();
+    if (more) commit(sm);
+    __syncthreads();
+    const int t = sm;
+    sm = s0;
+    s0 = sp;
+    sp = t;
   }
 }
 
-// calcRHS, src/mod_calcRHS.cc:27-29.
+// calcRHS, src/mod_calcRHS.cc:27-29.  w(k-1) is carried from the previous plane.
 template <typename T>
-__global__ __launch_bounds__(kTx* kTy) void k_calc_rhs(Geom g, const T* __restrict__ u,
-                                                       const T* __restrict__ v,
-                                                       const T* __restrict__ w, T* __restrict__ p,
-                                                       const T* __restrict__ dzf_inv, T dti,
-                                                       T dtidxi, T dtidyi, int kchunk) {
-  const int i = blockIdx.x * kTx + threadIdx.x + 1;
-  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+__global__ __launch_bounds__(kRkTx* kRkTy) void k_calc_rhs(Geom g, const T* __restrict__ u,
+                                                           const T* __restrict__ v,
+                                                           const T* __restrict__ w,
+                                                           T* __restrict__ p,
+                                                           const T* __restrict__ dzf_inv, T dti,
+                                                           T dtidxi, T dtidyi, int kchunk,
+                                                           int ntx, int nty) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
+  const int kc = (int)(b / (unsigned)(ntx * nty));
+  const int i = ti * kRkTx + (int)(threadIdx.x % kRkTx) + 1;
+  const int j = tj * kRkTy + (int)(threadIdx.x / kRkTx) + 1;
   if (i > g.nx || j > g.ny) return;
-  const int k0 = blockIdx.z * kchunk + 1;
-  const int k1 = min(k0 + kchunk - 1, g.nz);
-  for (int k = k0; k <= k1; ++k) {
-    const long c = g.idx(i, j, k);
-    p[c] = (u[c] - u[c - 1]) * dtidxi + (v[c] - v[c - g.sx]) * dtidyi +
-           (w[c] - w[c - g.sxy]) * dti * dzf_inv[k];
+  const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
+  long c = g.idx(i, j, k0);
+  T wb = k0 <= k1 ? w[c - g.sxy] : T(0);
+  for (int k = k0; k <= k1; ++k, c += g.sxy) {
+    const T w0 = w[c];
+    p[c] = (u[c] - u[c - 1]) * dtidxi + (v[c] - v[c - g.sx]) * dtidyi + (w0 - wb) * dti * dzf_inv[k];
+    wb = w0;
   }
 }
 
-// correctVel, src/mod_calcVel.cc:272-274.
+// correctVel, src/mod_calcVel.cc:272-274.  p(k+1) becomes the next plane's p(k).
 template <typename T>
-__global__ __launch_bounds__(kTx* kTy) void k_correct(Geom g, const T* __restrict__ p,
-                                                      T* __restrict__ u, T* __restrict__ v,
-                                                      T* __restrict__ w,
-                                                      const T* __restrict__ dzc_inv, T dtdxi,
-                                                      T dtdyi, T dt, int kchunk) {
-  const int i = blockIdx.x * kTx + threadIdx.x + 1;
-  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+__global__ __launch_bounds__(kRkTx* kRkTy) void k_correct(Geom g, const T* __restrict__ p,
+                                                          T* __restrict__ u, T* __restrict__ v,
+                                                          T* __restrict__ w,
+                                                          const T* __restrict__ dzc_inv, T dtdxi,
+                                                          T dtdyi, T dt, int kchunk, int ntx,
+                                                          int nty) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
+  const int kc = (int)(b / (unsigned)(ntx * nty));
+  const int i = ti * kRkTx + (int)(threadIdx.x % kRkTx) + 1;
+  const int j = tj * kRkTy + (int)(threadIdx.x / kRkTx) + 1;
   if (i > g.nx || j > g.ny) return;
-  const int k0 = blockIdx.z * kchunk + 1;
-  const int k1 = min(k0 + kchunk - 1, g.nz);
-  for (int k = k0; k <= k1; ++k) {
-    const long c = g.idx(i, j, k);
-    const T p0 = p[c];
+  const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
+  long c = g.idx(i, j, k0);
+  T p0 = k0 <= k1 ? p[c] : T(0);
+  for (int k = k0; k <= k1; ++k, c += g.sxy) {
+    const T pt = p[c + g.sxy];
     u[c] -= (p[c + 1] - p0) * dtdxi;
     v[c] -= (p[c + g.sx] - p0) * dtdyi;
-    w[c] -= (p[c + g.sxy] - p0) * dt * dzc_inv[k];
+    w[c] -= (pt - p0) * dt * dzc_inv[k];
+    p0 = pt;
   }
 }
 
@@ -524,23 +619,43 @@ inline T* F(pll_context* c, int f) {
   return static_cast<T*>(c->fld[f]);
 }
 
+// Tiled launch shape of the stencil kernels: ntx*nty tiles of 64x8 columns,
+// each marching a chunk of k planes; about 2048 blocks in total.
+struct TileGrid {
+  int ntx, nty, nkc, kchunk;
+  unsigned blocks() const { return (unsigned)ntx * nty * nkc; }
+};
+
+inline TileGrid tile_grid(const pll_context* c) {
+  TileGrid t;
+  t.ntx = (c->g.nx + kRkTx - 1) / kRkTx;
+  t.nty = (c->g.ny + kRkTy - 1) / kRkTy;
+  const long cols = (long)t.ntx * t.nty;
+  long want = (2048 + cols - 1) / cols;
+  int kc = (int)((c->g.nz + want - 1) / want);
+  if (kc < 4) kc = 4;
+  if (kc > c->g.nz) kc = c->g.nz;
+  t.kchunk = kc;
+  t.nkc = (c->g.nz + kc - 1) / kc;
+  return t;
+}
+
 template <typename T>
 int rk_t(pll_context* c, int stage) {
-  const int kc = kchunk_for(c);
-  const dim3 grid = col_grid(c, kc), block(kTx, kTy);
+  const TileGrid t = tile_grid(c);
   const MeshT<T> m = mesh_of<T>(c);
   if (stage == 1) {
     TimerScope ts(c, K_RK1);
-    hipLaunchKernelGGL((k_rk<T, 1>), grid, block, 0, c->stream, c->g, F<T>(c, PLL_U),
-                       F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1), F<T>(c, PLL_V1),
-                       F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv, (T)c->re_inv, (T)c->dt,
-                       (T)c->u_crf, kc);
+    hipLaunchKernelGGL((k_rk<T, 1>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
+                       F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
+                       F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
+                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty);
   } else {
     TimerScope ts(c, K_RK2);
-    hipLaunchKernelGGL((k_rk<T, 2>), grid, block, 0, c->stream, c->g, F<T>(c, PLL_U1),
-                       F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U), F<T>(c, PLL_V),
-                       F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv, (T)c->re_inv, (T)c->dt,
-                       (T)c->u_crf, kc);
+    hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
+                       F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
+                       F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
+                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty);
   }
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
@@ -548,27 +663,27 @@ int rk_t(pll_context* c, int stage) {
 
 template <typename T>
 int rhs_t(pll_context* c) {
-  const int kc = kchunk_for(c);
+  const TileGrid t = tile_grid(c);
   const double dti = 1.0 / c->dt;  // src/mod_calcRHS.cc:11-13
   const double dtidxi = dti * c->dx_inv;
   const double dtidyi = dti * c->dy_inv;
   TimerScope ts(c, K_RHS);
-  hipLaunchKernelGGL((k_calc_rhs<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+  hipLaunchKernelGGL((k_calc_rhs<T>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                      F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_P),
-                     mesh_of<T>(c).dzf_inv, (T)dti, (T)dtidxi, (T)dtidyi, kc);
+                     mesh_of<T>(c).dzf_inv, (T)dti, (T)dtidxi, (T)dtidyi, t.kchunk, t.ntx, t.nty);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
 
 template <typename T>
 int correct_t(pll_context* c) {
-  const int kc = kchunk_for(c);
+  const TileGrid t = tile_grid(c);
   const double dtdxi = c->dt * c->dx_inv;  // src/mod_calcVel.cc:256-258
   const double dtdyi = c->dt * c->dy_inv;
   TimerScope ts(c, K_CORRECT);
-  hipLaunchKernelGGL((k_correct<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+  hipLaunchKernelGGL((k_correct<T>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                      F<T>(c, PLL_P), F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W),
-                     mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt, kc);
+                     mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt, t.kchunk, t.ntx, t.nty);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }

@@ -105,6 +105,9 @@ struct pll_context {
   void* d_tw_x = nullptr;
   void* d_tw_y = nullptr;
   void* d_rtw_x = nullptr;  // exp(-2 pi i k/nx), k = 0..nx/2
+  void* d_twl_x = nullptr;  // exp(-2 pi i t/L), L = nx/2 (fixed-length kernels)
+  void* d_twl_y = nullptr;  // exp(-2 pi i t/L), L = ny
+  bool x_fixed = false, y_fixed = false;
   int x_batch = 1, y_tile = 2;
 
   // reductions

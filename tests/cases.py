@@ -14,6 +14,13 @@ CASES = {
     "r235": (24, 20, 12, 2 * math.pi, math.pi, 2.0, 1.5, 2800.0, 2e-3),
     "r3": (96, 48, 32, 2 * math.pi, math.pi, 2.0, 1.8, 10000.0, 5e-4),
     "tiny": (8, 6, 4, 1.3, 0.7, 2.0, 1.5, 100.0, 1e-3),
+    # sizes without a fixed-length transform kernel (generic Stockham path)
+    "gen": (40, 30, 16, 2 * math.pi, math.pi, 2.0, 1.5, 2800.0, 2e-3),
+    # the Re_tau~180 grid and thin slabs of the Re_tau~550 / ~1000 grids: the
+    # production transform lengths (x: 128, 384, 768 points; y: 192, 384, 768)
+    "c2": (256, 192, 128, 4 * math.pi, 2 * math.pi, 2.0, 1.5, 2800.0, 1e-3),
+    "c3slab": (768, 384, 8, 2 * math.pi, math.pi, 2.0, 1.8, 10000.0, 5e-4),
+    "c4slab": (1536, 768, 4, 2 * math.pi, math.pi, 2.0, 2.0, 20000.0, 2.5e-4),
 }
 
 

@@ -116,7 +116,7 @@ def test_calc_rhs_and_correct_bitexact(ctxs, name):
         np.testing.assert_array_equal(interior(ch.download(fld)), interior(arr), err_msg=fld)
 
 
-@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("name", SMALL + ["gen", "c3slab", "c4slab", "c2"])
 def test_fft_stages(ctxs, name):
     _print_case("fft stages", name)
     ch, o = ctxs(name)
@@ -146,7 +146,7 @@ def test_fft_stages(ctxs, name):
     assert rel_err(interior(ch.download("p")), interior(q)) < 1e-13
 
 
-@pytest.mark.parametrize("name", SMALL)
+@pytest.mark.parametrize("name", SMALL + ["gen", "c3slab"])
 def test_poisson_matches_oracle(ctxs, name):
     _print_case("poisson", name)
     ch, o = ctxs(name)
@@ -198,7 +198,7 @@ def test_init_flow_matches_oracle(ctxs, name):
             assert rel_err(interior(ch.download(fld)), interior(arr)) < 1e-14, (prof, fld)
 
 
-@pytest.mark.parametrize("name,steps", [("c1", 10), ("r235", 10), ("r3", 5)])
+@pytest.mark.parametrize("name,steps", [("c1", 10), ("r235", 10), ("r3", 5), ("gen", 10)])
 def test_full_step_parity(name, steps):
     """10 RK2 steps from the same seeded state: u, v, w and p - <p> to 1e-11."""
     _print_case("full step", name, steps)
