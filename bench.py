@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed"])
+    ap.add_argument("--fp64-only", action="store_true", help="skip the FP32-mixed measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true",
@@ -115,6 +115,59 @@ def cpu_baseline(wl, steps):
     }
 
 
+def measure(wl, precision, steps, warmup):
+    """Time `steps` RK2 steps after `warmup`, then re-run them under the per-kernel timers."""
+    ch, npts = build_channel(wl, precision)
+    print(f"# workload {wl} precision {precision}: {npts} points, "
+          f"{ch.device_bytes() / 1e9:.2f} GB on device", flush=True)
+    ch.step(warmup)
+    ch.synchronize()
+    t0 = time.perf_counter()
+    ch.step(steps)
+    ch.synchronize()
+    sec = time.perf_counter() - t0
+    ms_step = 1e3 * sec / steps
+    print(f"# {precision}: timed {steps} steps, {ms_step:.3f} ms/step", flush=True)
+    ch.timing(True)
+    ch.timing_reset()
+    ch.step(steps)
+    ch.synchronize()
+    times = ch.timing_read()
+    ch.timing(False)
+    mean_u = ch.Mean("u")
+    div = ch.calcMaxDiv()[0]
+    ch.close()
+    scale = 1.0 if precision == "fp64" else 0.5
+    kernels = {}
+    total = sum(ms for ms, n in times.values() if n)
+    for name, (ms, n) in times.items():
+        if n == 0:
+            continue
+        avg_ms = ms / n
+        entry = {"avg_us": round(avg_ms * 1e3, 2), "launches": n, "share": round(ms / total, 4)}
+        bpp = BYTES_PER_POINT_FP64.get(name)
+        if bpp is not None:
+            gbs = bpp * scale * npts / (avg_ms * 1e-3) / 1e9
+            entry["GBps"] = round(gbs, 1)
+            entry["frac"] = round(gbs / HBM_PEAK_GBS, 4)
+        kernels[name] = entry
+    dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: times[k][0])
+    d_avg_s = times[dom][0] / times[dom][1] * 1e-3
+    achieved = BYTES_PER_POINT_FP64[dom] * scale * npts / d_avg_s / 1e9
+    step_gbs = STEP_BYTES_FP64 * scale * npts / (ms_step * 1e-3) / 1e9
+    return {
+        "value": round(npts * steps / sec / 1e6, 2),
+        "ms_per_step": round(ms_step, 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "step_achieved": round(step_gbs, 1),
+                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                     "bytes_per_point_step": STEP_BYTES_FP64 * scale},
+        "kernels": kernels,
+        "check": {"bulk_u": mean_u, "max_div": div},
+    }
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -124,71 +177,34 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(the multi-GPU sharding is a design, see DESIGN.md)"}))
         sys.exit(2)
-    ch, npts = build_channel(args.workload, args.precision)
-    print(f"# workload {args.workload} precision {args.precision}: {npts} points, "
-          f"{ch.device_bytes() / 1e9:.2f} GB on device", flush=True)
-    ch.step(args.warmup)
-    ch.synchronize()
-    t0 = time.perf_counter()
-    ch.step(args.steps)
-    ch.synchronize()
-    sec = time.perf_counter() - t0
-    ms_step = 1e3 * sec / args.steps
-    value = npts * args.steps / sec / 1e6
-    print(f"# timed {args.steps} steps: {ms_step:.3f} ms/step", flush=True)
-
-    # per-kernel HIP-event timings over a separate window of the same steps
-    ch.timing(True)
-    ch.timing_reset()
-    ch.step(args.steps)
-    ch.synchronize()
-    times = ch.timing_read()
-    ch.timing(False)
-    scale = 1.0 if args.precision == "fp64" else 0.5
-    kernels = {}
-    for name, (ms, n) in times.items():
-        if n == 0:
-            continue
-        avg_ms = ms / n
-        bpp = BYTES_PER_POINT_FP64.get(name)
-        entry = {"avg_us": round(avg_ms * 1e3, 2), "launches": n, "share": None}
-        if bpp is not None:
-            gbs = bpp * scale * npts / (avg_ms * 1e-3) / 1e9
-            entry["GBps"] = round(gbs, 1)
-            entry["frac"] = round(gbs / HBM_PEAK_GBS, 4)
-        kernels[name] = entry
-    total = sum(ms for ms, n in times.values() if n)
-    for name, (ms, n) in times.items():
-        if n:
-            kernels[name]["share"] = round(ms / total, 4)
-    dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: times[k][0])
-    d_avg_s = times[dom][0] / times[dom][1] * 1e-3
-    d_bytes = BYTES_PER_POINT_FP64[dom] * scale * npts
-    achieved = d_bytes / d_avg_s / 1e9
-    step_frac = STEP_BYTES_FP64 * scale * npts / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
-
+    main64 = measure(args.workload, "fp64", args.steps, args.warmup)
     result = {
         "metric": "Mpoint-steps/s per RK2 step (FP64 vs FP32-mixed) + % HBM roofline, 1 GPU",
-        "value": round(value, 2),
+        "value": main64["value"],
         "unit": "Mpoint-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": main64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic (seeded log profile + 10% uniform noise)",
         "config": {"workload": args.workload, "grid": list(WORKLOADS[args.workload][:3]),
-                   "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "step_frac": round(step_frac, 4),
-                     "bytes_per_point_step": STEP_BYTES_FP64 * scale},
-        "kernels": kernels,
+                   "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": main64["roofline"],
+        "kernels": main64["kernels"],
+        "check": main64["check"],
     }
-    ch.close()
+    if not args.fp64_only:
+        mixed = measure(args.workload, "fp32_mixed", args.steps, args.warmup)
+        result["fp32_mixed"] = {
+            "value": mixed["value"], "ms_per_step": mixed["ms_per_step"],
+            "dtype": "f32 fields/stencils/FFTs, f64 tridiagonal arithmetic and bulk reduction",
+            "speedup_vs_fp64": round(mixed["value"] / main64["value"], 3),
+            "roofline": mixed["roofline"], "kernels": mixed["kernels"], "check": mixed["check"],
+        }
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", flush=True)
         result["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps)

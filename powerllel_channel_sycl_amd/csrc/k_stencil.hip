@@ -123,13 +123,26 @@ constexpr int kRkTy = 8;   // tile rows along y
 // source fields of planes k-1, k, k+1 (tile plus a one-cell halo) sit in an
 // LDS ring; plane k+2 is fetched into registers while plane k is computed and
 // replaces plane k-1 afterwards, so every source value leaves HBM once per block.
+//
+// Inside a multi-step run the previous step's forcing and final boundary
+// update are folded into the loads (`fold`): `force` (device scalars, one per
+// forced component) is added to every source value, and every halo cell is
+// rebuilt from interior cells by the reference's closed-form rules -- x/y
+// halos are periodic images, the z ghost planes are 2*bc - (u + f) for u,
+// -(v + f) for v and 0 for w, and w on plane nz is 0
+// (src/mod_updateBound.cc:11-144) -- so the result is bit-identical to forcing,
+// updating the boundary and then running the stage.  Stage 2 adds the
+// deferred forcing to the point value of u it reads.
 template <typename T, int STAGE>
 __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict__ su,
                                                      const T* __restrict__ sv,
                                                      const T* __restrict__ sw, T* __restrict__ du,
                                                      T* __restrict__ dv, T* __restrict__ dw,
                                                      MeshT<T> m, T dxi, T dyi, T rei, T dt,
-                                                     T ucrf, int kchunk, int ntx, int nty) {
+                                                     T ucrf, int kchunk, int ntx, int nty,
+                                                     int fold, int defer, T bc_u,
+                                                     const double* __restrict__ force,
+                                                     int forced_mask) {
   constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY, NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
   __shared__ T sh[3][3][PL];  // [field][ring slot][tile cell]
@@ -141,25 +154,50 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int i = i0 + tx + 1, j = j0 + ty + 1;
   const bool active = i <= g.nx && j <= g.ny;
+  // deferred forcing of the three components (applied only where forced):
+  // to every loaded value when folding, to the point value of stage 2 when deferring
+  const bool any = fold || defer;
+  const bool fu = any && (forced_mask & 1), fv = any && (forced_mask & 2),
+             fw = any && (forced_mask & 4);
+  const T f_u = fu ? (T)force[0] : T(0), f_v = fv ? (T)force[1] : T(0),
+          f_w = fw ? (T)force[2] : T(0);
 
-  long gaddr[NPF];  // plane-relative address of this thread's tile cells (-1: outside)
+  long gaddr[NPF];  // plane-relative source address of this thread's tile cells (-1: outside)
   int cell[NPF];
 #pragma unroll
   for (int n = 0; n < NPF; ++n) {
     const int e = threadIdx.x + n * NT;
-    const int gx = i0 + e % SX, gy = j0 + e / SX;
+    int gx = i0 + e % SX, gy = j0 + e / SX;
     cell[n] = e;
-    gaddr[n] = (e < PL && gx <= g.nx + 1 && gy <= g.ny + 1) ? gx + g.sx * gy : -1;
+    const bool inside = e < PL && gx <= g.nx + 1 && gy <= g.ny + 1;
+    if (fold) {  // halo cells read their periodic image
+      gx = gx == 0 ? g.nx : (gx == g.nx + 1 ? 1 : gx);
+      gy = gy == 0 ? g.ny : (gy == g.ny + 1 ? 1 : gy);
+    }
+    gaddr[n] = inside ? gx + g.sx * gy : -1;
   }
   T pre[3][NPF];
   auto fetch = [&](int kk) {
+    const bool ghost = fold && (kk == 0 || kk == g.nz + 1);
+    const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
+    const bool wzero = fold && (kk == 0 || kk >= g.nz);
 #pragma unroll
     for (int n = 0; n < NPF; ++n)
       if (gaddr[n] >= 0) {
-        const long a = gaddr[n] + g.sxy * kk;
-        pre[0][n] = su[a];
-        pre[1][n] = sv[a];
-        pre[2][n] = sw[a];
+        const long a = gaddr[n] + g.sxy * ks;
+        T qu = su[a], qv = sv[a], qw = wzero ? T(0) : sw[a];
+        if (fold) {
+          if (fu) qu = qu + f_u;
+          if (fv) qv = qv + f_v;
+          if (fw && !wzero) qw = qw + f_w;
+        }
+        if (ghost) {
+          qu = T(2) * bc_u - qu;  // src/mod_updateBound.cc:92,123
+          qv = T(2) * T(0) - qv;
+        }
+        pre[0][n] = qu;
+        pre[1][n] = qv;
+        pre[2][n] = qw;
       }
   };
   auto commit = [&](int slot) {
@@ -196,9 +234,15 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
         dv[a] = V(0, 0, 0) + dt * tv;
         dw[a] = W(0, 0, 0) + dt * tw;
       } else {
-        du[a] = (U(0, 0, 0) + dt * tu + du[a]) * T(0.5);
-        dv[a] = (V(0, 0, 0) + dt * tv + dv[a]) * T(0.5);
-        dw[a] = (W(0, 0, 0) + dt * tw + dw[a]) * T(0.5);
+        T ou = du[a], ov = dv[a], ow = dw[a];
+        if (defer) {
+          if (fu) ou = ou + f_u;
+          if (fv) ov = ov + f_v;
+          if (fw) ow = (k == g.nz) ? T(0) : ow + f_w;
+        }
+        du[a] = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
+        dv[a] = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
+        dw[a] = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
       }
     }
     __syncthreads();
@@ -237,28 +281,45 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_calc_rhs(Geom g, const T* __re
 }
 
 // correctVel, src/mod_calcVel.cc:272-274.  p(k+1) becomes the next plane's p(k).
-template <typename T>
+// With ACC the block also sums the corrected u times dzflzi(k) (FP64) into
+// partial[block] -- the first half of the bulk mean that forceVel needs.
+template <typename T, bool ACC>
 __global__ __launch_bounds__(kRkTx* kRkTy) void k_correct(Geom g, const T* __restrict__ p,
                                                           T* __restrict__ u, T* __restrict__ v,
                                                           T* __restrict__ w,
                                                           const T* __restrict__ dzc_inv, T dtdxi,
                                                           T dtdyi, T dt, int kchunk, int ntx,
-                                                          int nty) {
+                                                          int nty, const double* __restrict__ dzflzi,
+                                                          double* __restrict__ partial) {
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
   const int kc = (int)(b / (unsigned)(ntx * nty));
   const int i = ti * kRkTx + (int)(threadIdx.x % kRkTx) + 1;
   const int j = tj * kRkTy + (int)(threadIdx.x / kRkTx) + 1;
-  if (i > g.nx || j > g.ny) return;
   const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
-  long c = g.idx(i, j, k0);
-  T p0 = k0 <= k1 ? p[c] : T(0);
-  for (int k = k0; k <= k1; ++k, c += g.sxy) {
-    const T pt = p[c + g.sxy];
-    u[c] -= (p[c + 1] - p0) * dtdxi;
-    v[c] -= (p[c + g.sx] - p0) * dtdyi;
-    w[c] -= (pt - p0) * dt * dzc_inv[k];
-    p0 = pt;
+  double acc = 0.0;
+  if (i <= g.nx && j <= g.ny && k0 <= k1) {
+    long c = g.idx(i, j, k0);
+    T p0 = p[c];
+    for (int k = k0; k <= k1; ++k, c += g.sxy) {
+      const T pt = p[c + g.sxy];
+      const T un = u[c] - (p[c + 1] - p0) * dtdxi;
+      u[c] = un;
+      v[c] -= (p[c + g.sx] - p0) * dtdyi;
+      w[c] -= (pt - p0) * dt * dzc_inv[k];
+      if (ACC) acc += (double)un * dzflzi[k];
+      p0 = pt;
+    }
+  }
+  if (ACC) {
+    __shared__ double sh[kRkTx * kRkTy];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kRkTx * kRkTy / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[b] = sh[0];
   }
 }
 
@@ -279,14 +340,14 @@ __device__ __forceinline__ double block_sum(double s, double* sh) {
 
 template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_mean_partial(Geom g, const T* __restrict__ f,
-                                                              const T* __restrict__ dzflzi,
+                                                              const double* __restrict__ dzflzi,
                                                               double* __restrict__ partial,
                                                               int rows_per_chunk) {
   __shared__ double sh[kRedThreads];
   const int k = blockIdx.y + 1;
   const int j0 = blockIdx.x * rows_per_chunk;
   const int j1 = min(j0 + rows_per_chunk, g.ny);
-  const double wk = (double)dzflzi[k];
+  const double wk = dzflzi[k];
   double s = 0.0;
   for (int j = j0; j < j1; ++j) {
     const T* row = f + g.idx(1, j + 1, k);
@@ -640,22 +701,30 @@ inline TileGrid tile_grid(const pll_context* c) {
   return t;
 }
 
+inline int forced_mask(const pll_context* c) {
+  return (c->is_forced[0] ? 1 : 0) | (c->is_forced[1] ? 2 : 0) | (c->is_forced[2] ? 4 : 0);
+}
+
 template <typename T>
-int rk_t(pll_context* c, int stage) {
+int rk_t(pll_context* c, int stage, int fold) {
   const TileGrid t = tile_grid(c);
   const MeshT<T> m = mesh_of<T>(c);
+  const T bc_u = (T)(0.0 - c->u_crf);  // src/mod_updateBound.cc:80
+  const double* force = c->d_scalar + 3;
   if (stage == 1) {
     TimerScope ts(c, K_RK1);
     hipLaunchKernelGGL((k_rk<T, 1>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                        F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
                        F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
-                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty);
+                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
+                       bc_u, force, forced_mask(c));
   } else {
     TimerScope ts(c, K_RK2);
     hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                        F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
                        F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
-                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty);
+                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, 0, fold,
+                       bc_u, force, forced_mask(c));
   }
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
@@ -676,14 +745,44 @@ int rhs_t(pll_context* c) {
 }
 
 template <typename T>
-int correct_t(pll_context* c) {
+int correct_t(pll_context* c, bool with_mean) {
   const TileGrid t = tile_grid(c);
   const double dtdxi = c->dt * c->dx_inv;  // src/mod_calcVel.cc:256-258
   const double dtdyi = c->dt * c->dy_inv;
-  TimerScope ts(c, K_CORRECT);
-  hipLaunchKernelGGL((k_correct<T>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
-                     F<T>(c, PLL_P), F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W),
-                     mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt, t.kchunk, t.ntx, t.nty);
+  const double* dzflzi = c->d_mesh64 + 3 * (c->g.nz + 2);
+  if (with_mean && (long)t.blocks() > c->npartial)
+    return fail(PLL_ERR_STATE, "reduction scratch smaller than the projection grid");
+  {
+    TimerScope ts(c, K_CORRECT);
+    if (with_mean)
+      hipLaunchKernelGGL((k_correct<T, true>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream,
+                         c->g, F<T>(c, PLL_P), F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W),
+                         mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt, t.kchunk, t.ntx,
+                         t.nty, dzflzi, c->d_partial);
+    else
+      hipLaunchKernelGGL((k_correct<T, false>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0,
+                         c->stream, c->g, F<T>(c, PLL_P), F<T>(c, PLL_U), F<T>(c, PLL_V),
+                         F<T>(c, PLL_W), mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt,
+                         t.kchunk, t.ntx, t.nty, dzflzi, c->d_partial);
+  }
+  if (with_mean) {
+    // forceVel's bulk mean and force of u (src/mod_calcVel.cc:287,297)
+    TimerScope ts(c, K_MEAN_FINAL);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kRedThreads), 0, c->stream, c->d_partial,
+                       (long)t.blocks(), (double)c->g.nx, (double)c->g.ny, c->d_scalar + 0,
+                       c->vel_force[0], c->d_scalar + 3);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+// forceVel's update of u with the force already on the device.
+template <typename T>
+int add_force_u_t(pll_context* c) {
+  const int kc = kchunk_for(c);
+  TimerScope ts(c, K_FORCE);
+  hipLaunchKernelGGL((k_add_force<T>), col_grid(c, kc), dim3(kTx, kTy), 0, c->stream, c->g,
+                     F<T>(c, PLL_U), c->d_scalar + 3, kc);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
@@ -696,7 +795,8 @@ int mean_t(pll_context* c, int field, double nxg, double nyg, double* d_out, dou
   {
     TimerScope ts(c, K_MEAN_PARTIAL);
     hipLaunchKernelGGL((k_mean_partial<T>), dim3(chunks, c->g.nz), dim3(kRedThreads), 0,
-                       c->stream, c->g, F<T>(c, field), mesh_of<T>(c).dzflzi, c->d_partial, rpc);
+                       c->stream, c->g, F<T>(c, field), c->d_mesh64 + 3 * (c->g.nz + 2),
+                       c->d_partial, rpc);
   }
   {
     TimerScope ts(c, K_MEAN_FINAL);
@@ -835,12 +935,15 @@ int nan_t(pll_context* c, int field, int* passed) {
 #define PLL_DISPATCH(c, call_fp64, call_fp32) \
   ((c)->prec == PLL_FP64 ? (call_fp64) : (call_fp32))
 
-int launch_rk(pll_context* c, int stage) {
-  return PLL_DISPATCH(c, rk_t<double>(c, stage), rk_t<float>(c, stage));
+int launch_rk(pll_context* c, int stage, int fold) {
+  return PLL_DISPATCH(c, rk_t<double>(c, stage, fold), rk_t<float>(c, stage, fold));
 }
 int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
-int launch_correct_vel(pll_context* c) {
-  return PLL_DISPATCH(c, correct_t<double>(c), correct_t<float>(c));
+int launch_correct_vel(pll_context* c, bool with_mean) {
+  return PLL_DISPATCH(c, correct_t<double>(c, with_mean), correct_t<float>(c, with_mean));
+}
+int launch_add_force_u(pll_context* c) {
+  return PLL_DISPATCH(c, add_force_u_t<double>(c), add_force_u_t<float>(c));
 }
 int launch_mean(pll_context* c, int field, double nxg, double nyg, double* d_out) {
   return PLL_DISPATCH(c, mean_t<double>(c, field, nxg, nyg, d_out, 0.0, nullptr),

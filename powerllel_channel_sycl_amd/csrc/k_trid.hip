@@ -27,6 +27,10 @@ constexpr int kTy = 4;
 // perm index -> compact eigenvalue index: 0 -> 0, 1 -> 1 (Nyquist), 2m/2m+1 -> m+1.
 __host__ __device__ __forceinline__ int compact_index(int i) { return i <= 1 ? i : (i >> 1) + 1; }
 
+// Loads are issued a chunk of D planes ahead of the sequential recurrence, so
+// each sweep waits on HBM latency once per D planes rather than once per plane.
+constexpr int kD = 8;
+
 template <typename T>
 __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __restrict__ var,
                                                    const double* __restrict__ a,
@@ -38,27 +42,79 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
   if (i >= nx || j >= ny) return;
   const long nxy = (long)nx * ny;
   const long P = (long)cxn * cyn;
-  const double* pv = piv + (long)compact_index(j) * cxn + compact_index(i);
-  T* col = var + (long)j * nx + i;
-  // forward elimination, src/mod_poisson_solver.cc:334-339
-  double prev = (double)col[0];
-  for (int k = 1; k < nz; ++k) {
-    const double bv = pv[(k - 1) * P];
-    const double a_tmp = a[k] / bv;
-    const double cur = (double)col[k * nxy] - a_tmp * prev;
-    col[k * nxy] = (T)cur;
-    prev = cur;
+  // plane bases are uniform across the block; each lane keeps one 32-bit offset
+  const int off = j * nx + i;
+  const int poff = compact_index(j) * cxn + compact_index(i);
+  auto V = [&](int k) -> T& { return (var + (long)k * nxy)[off]; };
+  auto B = [&](int k) -> double { return (piv + (long)k * P)[poff]; };
+
+  // forward elimination, src/mod_poisson_solver.cc:334-339:
+  // var[k] -= (a[k] / b[k-1]) * var[k-1], k = 1..nz-1
+  double prev = (double)V(0);
+  T xv[2][kD];
+  double xb[2][kD];
+  auto load_fwd = [&](int buf, int kb) {
+#pragma unroll
+    for (int d = 0; d < kD; ++d) {
+      const int k = kb + d;
+      if (k < nz) {
+        xv[buf][d] = V(k);
+        xb[buf][d] = B(k - 1);
+      }
+    }
+  };
+  auto run_fwd = [&](int buf, int kb) {
+#pragma unroll
+    for (int d = 0; d < kD; ++d) {
+      const int k = kb + d;
+      if (k < nz) {
+        const double a_tmp = a[k] / xb[buf][d];
+        const double cur = (double)xv[buf][d] - a_tmp * prev;
+        V(k) = (T)cur;
+        prev = cur;
+      }
+    }
+  };
+  load_fwd(0, 1);
+  for (int kb = 1; kb < nz; kb += 2 * kD) {
+    if (kb + kD < nz) load_fwd(1, kb + kD);
+    run_fwd(0, kb);
+    if (kb + 2 * kD < nz) load_fwd(0, kb + 2 * kD);
+    if (kb + kD < nz) run_fwd(1, kb + kD);
   }
   // last row, src/mod_poisson_solver.cc:341-346
-  const double b_val = pv[(nz - 1) * P];
+  const double b_val = B(nz - 1);
   prev = (b_val != 0) ? prev / b_val : 0.0;
-  col[(nz - 1) * nxy] = (T)prev;
-  // back substitution, src/mod_poisson_solver.cc:348-354
-  for (int k = nz - 2; k >= 0; --k) {
-    const double bv = pv[k * P];
-    const double cur = ((double)col[k * nxy] - c[k] * prev) / bv;
-    col[k * nxy] = (T)cur;
-    prev = cur;
+  V(nz - 1) = (T)prev;
+  // back substitution, src/mod_poisson_solver.cc:348-354:
+  // var[k] = (var[k] - c[k] * var[k+1]) / b[k], k = nz-2..0
+  auto load_bwd = [&](int buf, int kt) {
+#pragma unroll
+    for (int d = 0; d < kD; ++d) {
+      const int k = kt - d;
+      if (k >= 0) {
+        xv[buf][d] = V(k);
+        xb[buf][d] = B(k);
+      }
+    }
+  };
+  auto run_bwd = [&](int buf, int kt) {
+#pragma unroll
+    for (int d = 0; d < kD; ++d) {
+      const int k = kt - d;
+      if (k >= 0) {
+        const double cur = ((double)xv[buf][d] - c[k] * prev) / xb[buf][d];
+        V(k) = (T)cur;
+        prev = cur;
+      }
+    }
+  };
+  load_bwd(0, nz - 2);
+  for (int kt = nz - 2; kt >= 0; kt -= 2 * kD) {
+    if (kt - kD >= 0) load_bwd(1, kt - kD);
+    run_bwd(0, kt);
+    if (kt - 2 * kD >= 0) load_bwd(0, kt - 2 * kD);
+    if (kt - kD >= 0) run_bwd(1, kt - kD);
   }
 }
 

@@ -1,5 +1,6 @@
 // pll_capi.hip -- the C-ABI of include/pll.h: context lifetime, field
 // transfer, and one entry point per reference routine of the hot path.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -253,7 +254,7 @@ int create_impl(const pll_config* cfg, const double* dzf_global, pll_context** o
   int chunks = 1;
   while (chunks * 2 <= c->g.ny && (long)chunks * c->g.nz < 2048) chunks *= 2;
   c->mean_chunks = chunks;
-  c->npartial = (long)chunks * c->g.nz;
+  c->npartial = std::max<long>((long)chunks * c->g.nz, 1L << 15);
   if ((e = hipMalloc(&c->d_partial, c->npartial * sizeof(double))) != hipSuccess)
     return bail(hip_fail(e, "hipMalloc(partial)"));
   if ((e = hipMalloc(&c->d_scalar, 64 * sizeof(double))) != hipSuccess)
@@ -279,18 +280,31 @@ int poisson_impl(pll_context* c) {
   return launch_fft_x_bwd(c);
 }
 
-int step_impl(pll_context* c) {
+// One step of src/main.cc:116-154.  When only u is forced (or nothing is),
+// the bulk mean is reduced inside correctVel, and within a multi-step run the
+// force update and the final boundary update of a step (:149-153) are folded
+// into the next step's RK loads (`fold`); the last step of a run applies them
+// explicitly, so every run ends in exactly the reference's state.
+int step_impl(pll_context* c, bool first, bool last) {
+  const bool fused = !c->is_forced[1] && !c->is_forced[2];
+  const int fold = (fused && !first) ? 1 : 0;
   int rc;
-  if ((rc = launch_rk(c, 1))) return rc;          // src/main.cc:117
+  if ((rc = launch_rk(c, 1, fold))) return rc;    // src/main.cc:117
   if ((rc = launch_bc_vel(c, 1))) return rc;      // :121
-  if ((rc = launch_rk(c, 2))) return rc;          // :125
+  if ((rc = launch_rk(c, 2, fold))) return rc;    // :125
   if ((rc = launch_bc_vel(c, 0))) return rc;      // :129
   if ((rc = launch_calc_rhs(c))) return rc;       // :133
   if ((rc = poisson_impl(c))) return rc;          // :137
   if ((rc = launch_bc_p(c))) return rc;           // :141
-  if ((rc = launch_correct_vel(c))) return rc;    // :145
-  if ((rc = launch_force(c))) return rc;          // :149
-  return launch_bc_vel(c, 0);                     // :153
+  if (!fused) {
+    if ((rc = launch_correct_vel(c, false))) return rc;  // :145
+    if ((rc = launch_force(c))) return rc;               // :149
+    return launch_bc_vel(c, 0);                          // :153
+  }
+  if ((rc = launch_correct_vel(c, c->is_forced[0] != 0))) return rc;  // :145 (+ mean of :149)
+  if (!last) return PLL_OK;
+  if (c->is_forced[0] && (rc = launch_add_force_u(c))) return rc;     // :149
+  return launch_bc_vel(c, 0);                                          // :153
 }
 
 }  // namespace
@@ -469,17 +483,17 @@ int pll_transform2crf(pll_context* c, double vel_crf) {
 
 int pll_time_int_vel_rk1(pll_context* c) {
   if (!c) return fail(PLL_ERR_ARG, "null context");
-  return launch_rk(c, 1);
+  return launch_rk(c, 1, 0);
 }
 
 int pll_time_int_vel_rk2(pll_context* c) {
   if (!c) return fail(PLL_ERR_ARG, "null context");
-  return launch_rk(c, 2);
+  return launch_rk(c, 2, 0);
 }
 
 int pll_correct_vel(pll_context* c) {
   if (!c) return fail(PLL_ERR_ARG, "null context");
-  return launch_correct_vel(c);
+  return launch_correct_vel(c, false);
 }
 
 int pll_force_vel(pll_context* c, double mean_out[3]) {
@@ -552,7 +566,7 @@ int pll_step(pll_context* c, int nsteps) {
   if (!c) return fail(PLL_ERR_ARG, "null context");
   if (nsteps < 0) return fail(PLL_ERR_ARG, "nsteps must be >= 0");
   for (int s = 0; s < nsteps; ++s) {
-    int rc = step_impl(c);
+    int rc = step_impl(c, s == 0, s == nsteps - 1);
     if (rc) return rc;
   }
   return PLL_OK;

@@ -159,9 +159,14 @@ struct TimerScope {
 
 // ---- launchers (implemented in the .hip files) -------------------------------------
 // stencils and pointwise kernels (k_stencil.hip)
-int launch_rk(pll_context* c, int stage);
+// fold: stage 1 applies the previous step's deferred force and rebuilds the
+// halos on load; stage 2 adds the deferred force to its point value of u.
+int launch_rk(pll_context* c, int stage, int fold);
 int launch_calc_rhs(pll_context* c);
-int launch_correct_vel(pll_context* c);
+// with_mean: also reduce the bulk mean of the corrected u and store the force
+// of u (vel_force[0] - mean) on the device.
+int launch_correct_vel(pll_context* c, bool with_mean);
+int launch_add_force_u(pll_context* c);
 int launch_mean(pll_context* c, int field, double nxg, double nyg, double* d_out);
 int launch_force(pll_context* c);
 int launch_bc_vel(pll_context* c, int which);

@@ -221,3 +221,55 @@ def test_full_step_parity(name, steps):
         umax = np.max(np.abs(interior(u)))
         dzmin = np.min(o.mesh()["dzf"])
         assert div < 1e-12 * umax / dzmin, div
+
+
+@pytest.mark.parametrize("name,precision", [("r3", "fp64"), ("c1", "fp64"), ("r3", "fp32_mixed")])
+def test_multistep_folding_is_bitexact(name, precision):
+    """pll_step(n) folds each step's forcing and final boundary update into the
+    next step's loads; it must give the same bits as n single steps."""
+    _print_case("folding", name, precision)
+    o = make_oracle(name)
+    u, v, w, p = (o.field() for _ in range(4))
+    o.init_flow(u, v, w, "poi", noise_intensity=0.1)
+    o.update_bound_vel(u, v, w)
+    o.update_bound_p(p)
+    outs = []
+    for mode in ("fused", "single"):
+        with make_channel(name, precision=precision) as ch:
+            for fld, arr in zip(VEL + ("p",), (u, v, w, p)):
+                ch.upload(fld, arr)
+            if mode == "fused":
+                ch.step(6)
+            else:
+                for _ in range(6):
+                    ch.step(1)
+            outs.append([ch.download(f) for f in VEL + ("p",)])
+    for a, b, fld in zip(outs[0], outs[1], VEL + ("p",)):
+        np.testing.assert_array_equal(a, b, err_msg=fld)
+
+
+def test_entry_point_sequence_matches_step():
+    """The reference's call sequence through the individual entry points
+    (src/main.cc:116-154) matches pll_step to round-off."""
+    name = "r3"
+    _print_case("entry sequence", name)
+    o = make_oracle(name)
+    u, v, w, p = (o.field() for _ in range(4))
+    o.init_flow(u, v, w, "poi", noise_intensity=0.1)
+    o.update_bound_vel(u, v, w)
+    o.update_bound_p(p)
+    res = []
+    for mode in ("step", "calls"):
+        with make_channel(name) as ch:
+            for fld, arr in zip(VEL + ("p",), (u, v, w, p)):
+                ch.upload(fld, arr)
+            for _ in range(3):
+                if mode == "step":
+                    ch.step(1)
+                else:
+                    ch.timeIntVelRK1(); ch.updateBoundVel(1); ch.timeIntVelRK2(); ch.updateBoundVel(0)
+                    ch.calcRHS(); ch.execute_poisson_solver(); ch.updateBoundP(); ch.correctVel()
+                    ch.forceVel(); ch.updateBoundVel(0)
+            res.append([ch.download(f) for f in VEL])
+    for a, b in zip(*res):
+        assert rel_err(a, b) < 1e-13
