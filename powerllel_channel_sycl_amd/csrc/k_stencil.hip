@@ -9,6 +9,9 @@
 #include "pll_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
+#include <utility>
+#include <vector>
 
 namespace pll {
 
@@ -108,6 +111,69 @@ struct PlaneAcc {
   }
 };
 
+// FP32-mixed form of the same operator, scaled by S (dt in stage 1, dt/2 in
+// stage 2): algebraically equal to S*(visc - conv) above, with the mesh factors
+// gathered into per-plane coefficients (RkCoef) and the viscous terms written
+// as second differences.  It is not bit-identical to the reference order and
+// is used only by the FP32-mixed path, whose error is measured against the
+// FP64 oracle.
+struct RkCoef {
+  float a1, a2, b1, b2;  // (u,v) face interpolation to k+1/2 and k-1/2
+  float zuv, cp, cm;     // S*0.25*dzf_inv(k); S*re_inv*dzc_inv(k|k-1)*dzf_inv(k)
+  float zw, dp, dm;      // S*0.25*dzc_inv(k); S*re_inv*dzf_inv(k+1|k)*dzc_inv(k)
+  float pad0, pad1;
+};
+struct RkUniform {
+  float cx, cy, vx, vy, crf;  // S*0.25*dxi, S*0.25*dyi, S*re_inv*dxi^2, S*re_inv*dyi^2, S*0.5*u_crf*dxi
+};
+
+template <bool CRF, class A>
+__device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W, const RkCoef& q,
+                                              const RkUniform& cu, float& iu, float& iv,
+                                              float& iw) {
+  {
+    const float q0 = U(0, 0, 0), qe = U(1, 0, 0), qw = U(-1, 0, 0), qn = U(0, 1, 0);
+    const float qs = U(0, -1, 0), qt = U(0, 0, 1), qb = U(0, 0, -1);
+    const float ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
+    const float at = q0 * q.a1 + qt * q.a2, ab = q0 * q.b1 + qb * q.b2;
+    const float vn = V(0, 0, 0) + V(1, 0, 0), vs = V(0, -1, 0) + V(1, -1, 0);
+    const float wt = W(0, 0, 0) + W(1, 0, 0), wb = W(0, 0, -1) + W(1, 0, -1);
+    float conv = cu.cx * (ae * ae - aw * aw) + cu.cy * (an * vn - as * vs) +
+                 q.zuv * (at * wt - ab * wb);
+    if (CRF) conv += cu.crf * (ae - aw);
+    const float q2 = q0 + q0;
+    iu = cu.vx * (qe + qw - q2) + cu.vy * (qn + qs - q2) + q.cp * (qt - q0) - q.cm * (q0 - qb) - conv;
+  }
+  {
+    const float q0 = V(0, 0, 0), qe = V(1, 0, 0), qw = V(-1, 0, 0), qn = V(0, 1, 0);
+    const float qs = V(0, -1, 0), qt = V(0, 0, 1), qb = V(0, 0, -1);
+    const float ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
+    const float at = q0 * q.a1 + qt * q.a2, ab = q0 * q.b1 + qb * q.b2;
+    const float ue = U(0, 0, 0) + U(0, 1, 0), uw = U(-1, 0, 0) + U(-1, 1, 0);
+    const float wt = W(0, 0, 0) + W(0, 1, 0), wb = W(0, 0, -1) + W(0, 1, -1);
+    float conv = cu.cx * (ue * ae - uw * aw) + cu.cy * (an * an - as * as) +
+                 q.zuv * (wt * at - wb * ab);
+    if (CRF) conv += cu.crf * (ae - aw);
+    const float q2 = q0 + q0;
+    iv = cu.vx * (qe + qw - q2) + cu.vy * (qn + qs - q2) + q.cp * (qt - q0) - q.cm * (q0 - qb) - conv;
+  }
+  {
+    const float q0 = W(0, 0, 0), qe = W(1, 0, 0), qw = W(-1, 0, 0), qn = W(0, 1, 0);
+    const float qs = W(0, -1, 0), qt = W(0, 0, 1), qb = W(0, 0, -1);
+    const float ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
+    const float at = q0 + qt, ab = q0 + qb;
+    const float ue = U(0, 0, 0) * q.a1 + U(0, 0, 1) * q.a2;
+    const float uw = U(-1, 0, 0) * q.a1 + U(-1, 0, 1) * q.a2;
+    const float vn = V(0, 0, 0) * q.a1 + V(0, 0, 1) * q.a2;
+    const float vs = V(0, -1, 0) * q.a1 + V(0, -1, 1) * q.a2;
+    float conv = cu.cx * (ue * ae - uw * aw) + cu.cy * (vn * an - vs * as) +
+                 q.zw * (at * at - ab * ab);
+    if (CRF) conv += cu.crf * (ae - aw);
+    const float q2 = q0 + q0;
+    iw = cu.vx * (qe + qw - q2) + cu.vy * (qn + qs - q2) + q.dp * (qt - q0) - q.dm * (q0 - qb) - conv;
+  }
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of logical blocks so neighbouring tiles share its L2.
 __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
@@ -142,7 +208,9 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
                                                      T ucrf, int kchunk, int ntx, int nty,
                                                      int fold, int defer, T bc_u,
                                                      const double* __restrict__ force,
-                                                     int forced_mask) {
+                                                     int forced_mask,
+                                                     const RkCoef* __restrict__ coef,
+                                                     RkUniform cu, int crf) {
   constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY, NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
   __shared__ T sh[3][3][PL];  // [field][ring slot][tile cell]
@@ -226,26 +294,245 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
       const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
       const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
       const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
-      T tu, tv, tw;
-      tendency<T>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw);
       const long a = g.idx(i, j, k);
-      if (STAGE == 1) {
-        du[a] = U(0, 0, 0) + dt * tu;
-        dv[a] = V(0, 0, 0) + dt * tv;
-        dw[a] = W(0, 0, 0) + dt * tw;
-      } else {
-        T ou = du[a], ov = dv[a], ow = dw[a];
+      T ou = T(0), ov = T(0), ow = T(0);
+      if (STAGE == 2) {
+        ou = du[a];
+        ov = dv[a];
+        ow = dw[a];
         if (defer) {
           if (fu) ou = ou + f_u;
           if (fv) ov = ov + f_v;
           if (fw) ow = (k == g.nz) ? T(0) : ow + f_w;
         }
-        du[a] = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
-        dv[a] = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
-        dw[a] = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
+      }
+      if constexpr (std::is_same<T, double>::value) {
+        // reference arithmetic, src/mod_calcVel.cc:81,103,125 and :194,216,238
+        T tu, tv, tw;
+        tendency<T>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw);
+        if (STAGE == 1) {
+          du[a] = U(0, 0, 0) + dt * tu;
+          dv[a] = V(0, 0, 0) + dt * tv;
+          dw[a] = W(0, 0, 0) + dt * tw;
+        } else {
+          du[a] = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
+          dv[a] = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
+          dw[a] = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
+        }
+      } else {
+        const RkCoef q = coef[k];
+        float iu, iv, iw;
+        if (crf)
+          tendency_fast<true>(U, V, W, q, cu, iu, iv, iw);
+        else
+          tendency_fast<false>(U, V, W, q, cu, iu, iv, iw);
+        if (STAGE == 1) {
+          du[a] = U(0, 0, 0) + iu;
+          dv[a] = V(0, 0, 0) + iv;
+          dw[a] = W(0, 0, 0) + iw;
+        } else {
+          du[a] = (U(0, 0, 0) + ou) * 0.5f + iu;
+          dv[a] = (V(0, 0, 0) + ov) * 0.5f + iv;
+          dw[a] = (W(0, 0, 0) + ow) * 0.5f + iw;
+        }
       }
     }
     __syncthreads();
+    if (more) commit(sm);
+    __syncthreads();
+    const int t = sm;
+    sm = s0;
+    s0 = sp;
+    sp = t;
+  }
+}
+
+// RK stage 2 fused with calcRHS (src/mod_calcVel.cc:140-242 then
+// src/mod_calcRHS.cc:27-29) for the step sequence.  Besides its 64x8 interior
+// the block evaluates u_new on the column i0 and v_new on the row j0 (the
+// backward neighbours calcRHS needs); at the domain edge these are computed
+// from periodic images of the stencil and equal the boundary copy bit for bit.
+// w_new(k-1) is carried in a register (a chunk first recomputes it for its
+// plane k0-1; below the wall it is the boundary value 0).  Loads rebuild the
+// u1 halos from interior cells (so the boundary update of u1 is not needed),
+// w_new on plane nz is the wall value 0 (the middle boundary update's only
+// effect on the interior), and the deferred force of the previous step is
+// added to the point values of u.  The new state goes to separate buffers
+// (nu, nv, nw): the backward neighbours read old values of cells that other
+// blocks update, so the update cannot be in place.
+template <typename T>
+__global__ __launch_bounds__(kRkTx* kRkTy) void k_rk2_rhs(
+    Geom g, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
+    const T* __restrict__ du, const T* __restrict__ dv, const T* __restrict__ dw,
+    T* __restrict__ nu_out, T* __restrict__ nv_out, T* __restrict__ nw_out, T* __restrict__ p, MeshT<T> m,
+    T dxi, T dyi, T rei, T dt, T ucrf, int kchunk, int ntx, int nty, int defer, T bc_u,
+    const double* __restrict__ force, int forced_mask, const RkCoef* __restrict__ coef,
+    RkUniform cu, int crf, T dti, T dtidxi, T dtidyi) {
+  constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 3, SY = TY + 3, PL = SX * SY, NT = TX * TY;
+  constexpr int NPF = (PL + NT - 1) / NT;
+  __shared__ T sh[3][3][PL];            // u1, v1, w1 ring; cell (cx, cy) = (i0-1+cx, j0-1+cy)
+  __shared__ T un[TY][TX + 1];          // u_new at i = i0 .. i0+TX of the tile rows
+  __shared__ T vn[TY + 1][TX];          // v_new at j = j0 .. j0+TY of the tile columns
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
+  const int kc = (int)(b / (unsigned)(ntx * nty));
+  const int i0 = ti * TX, j0 = tj * TY;
+  const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int i = i0 + tx + 1, j = j0 + ty + 1;
+  const bool active = i <= g.nx && j <= g.ny;
+  const bool fu = defer && (forced_mask & 1), fv = defer && (forced_mask & 2),
+             fw = defer && (forced_mask & 4);
+  const T f_u = fu ? (T)force[0] : T(0), f_v = fv ? (T)force[1] : T(0),
+          f_w = fw ? (T)force[2] : T(0);
+  auto wrapx = [&](int x) { return x < 1 ? x + g.nx : (x > g.nx ? x - g.nx : x); };
+  auto wrapy = [&](int y) { return y < 1 ? y + g.ny : (y > g.ny ? y - g.ny : y); };
+
+  long gaddr[NPF];
+  int cell[NPF];
+#pragma unroll
+  for (int n = 0; n < NPF; ++n) {
+    const int e = threadIdx.x + n * NT;
+    const int gx = i0 - 1 + e % SX, gy = j0 - 1 + e / SX;
+    cell[n] = e;
+    const bool inside = e < PL && gx <= g.nx + 1 && gy <= g.ny + 1;
+    gaddr[n] = inside ? wrapx(gx) + g.sx * wrapy(gy) : -1;
+  }
+  T pre[3][NPF];
+  auto fetch = [&](int kk) {  // u1, v1, w1 of plane kk with the boundary rules applied
+    const bool ghost = kk == 0 || kk == g.nz + 1;
+    const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
+    const bool wzero = kk == 0 || kk >= g.nz;
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        const long a = gaddr[n] + g.sxy * ks;
+        T qu = su[a], qv = sv[a];
+        const T qw = wzero ? T(0) : sw[a];
+        if (ghost) {
+          qu = T(2) * bc_u - qu;  // src/mod_updateBound.cc:92,123
+          qv = T(2) * T(0) - qv;
+        }
+        pre[0][n] = qu;
+        pre[1][n] = qv;
+        pre[2][n] = qw;
+      }
+  };
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        sh[0][slot][cell[n]] = pre[0][n];
+        sh[1][slot][cell[n]] = pre[1][n];
+        sh[2][slot][cell[n]] = pre[2][n];
+      }
+  };
+  // old (u, v, w) at a cell, with the deferred force; x/y halo cells wrap
+  auto old_u = [&](int x, int y, int kk) {
+    T q = du[g.idx(wrapx(x), wrapy(y), kk)];
+    return fu ? q + f_u : q;
+  };
+  auto old_v = [&](int x, int y, int kk) {
+    T q = dv[g.idx(wrapx(x), wrapy(y), kk)];
+    return fv ? q + f_v : q;
+  };
+  // the three stage-2 updates at tile cell (cx, cy) of plane kk; `which` selects components
+  auto update = [&](int sm, int s0, int sp, int cx, int cy, int kk, T& nu, T& nv, T& nw,
+                    T ou, T ov, T ow) {
+    const int c = cy * SX + cx;
+    const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
+    const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
+    const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
+    if constexpr (std::is_same<T, double>::value) {
+      T tu, tv, tw;
+      tendency<T>(U, V, W, m, kk, dxi, dyi, rei, ucrf, tu, tv, tw);
+      nu = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
+      nv = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
+      nw = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
+    } else {
+      const RkCoef q = coef[kk];
+      float iu, iv, iw;
+      if (crf)
+        tendency_fast<true>(U, V, W, q, cu, iu, iv, iw);
+      else
+        tendency_fast<false>(U, V, W, q, cu, iu, iv, iw);
+      nu = (U(0, 0, 0) + ou) * 0.5f + iu;
+      nv = (V(0, 0, 0) + ov) * 0.5f + iv;
+      nw = (W(0, 0, 0) + ow) * 0.5f + iw;
+    }
+  };
+  if (k0 > k1) return;
+  const int cxi = tx + 2, cyi = ty + 2;  // tile cell of the thread's interior point
+  int sm = 0, s0 = 1, sp = 2;
+  T wprev = T(0);  // w_new(i, j, k-1); 0 below the wall (w(0) = 0)
+  if (k0 > 1) {
+    // recompute w_new on plane k0-1 from u1 planes k0-2, k0-1, k0
+    fetch(k0 - 2);
+    commit(0);
+    fetch(k0 - 1);
+    commit(1);
+    fetch(k0);
+    commit(2);
+    __syncthreads();
+    if (active) {
+      T nu, nv, nw;
+      T ow = dw[g.idx(i, j, k0 - 1)];
+      if (fw) ow = (k0 - 1 == g.nz) ? T(0) : ow + f_w;
+      update(0, 1, 2, cxi, cyi, k0 - 1, nu, nv, nw, T(0), T(0), ow);
+      wprev = (k0 - 1 == g.nz) ? T(0) : nw;
+    }
+    __syncthreads();
+    fetch(k0 + 1);
+    commit(0);  // slot 0 held plane k0-2
+    __syncthreads();
+    sm = 1;
+    s0 = 2;
+    sp = 0;
+  } else {
+    fetch(k0 - 1);
+    commit(0);
+    fetch(k0);
+    commit(1);
+    fetch(k0 + 1);
+    commit(2);
+    __syncthreads();
+  }
+  for (int k = k0; k <= k1; ++k) {
+    const bool more = k + 2 <= k1 + 1;
+    if (more) fetch(k + 2);
+    T nw = T(0);
+    if (active) {
+      const long a = g.idx(i, j, k);
+      T ou = du[a], ov = dv[a], ow = dw[a];
+      if (fu) ou = ou + f_u;
+      if (fv) ov = ov + f_v;
+      if (fw) ow = (k == g.nz) ? T(0) : ow + f_w;
+      T nu, nv;
+      update(sm, s0, sp, cxi, cyi, k, nu, nv, nw, ou, ov, ow);
+      if (k == g.nz) nw = T(0);  // src/mod_updateBound.cc:137 (w(nz) = 0)
+      un[ty][tx + 1] = nu;
+      vn[ty + 1][tx] = nv;
+      nu_out[a] = nu;
+      nv_out[a] = nv;
+      nw_out[a] = nw;
+    }
+    // backward neighbours: v_new on row j0 (wave 0), u_new on column i0 (wave 1)
+    if (ty == 0 && i <= g.nx) {
+      T nu, nv, nw2;
+      update(sm, s0, sp, cxi, 1, k, nu, nv, nw2, T(0), old_v(i, j0, k), T(0));
+      vn[0][tx] = nv;
+    } else if (ty == 1 && tx < TY && j0 + tx + 1 <= g.ny) {
+      T nu, nv, nw2;
+      update(sm, s0, sp, 1, tx + 2, k, nu, nv, nw2, old_u(i0, j0 + tx + 1, k), T(0), T(0));
+      un[tx][0] = nu;
+    }
+    __syncthreads();
+    if (active) {
+      // calcRHS, src/mod_calcRHS.cc:27-29
+      p[g.idx(i, j, k)] = (un[ty][tx + 1] - un[ty][tx]) * dtidxi +
+                          (vn[ty + 1][tx] - vn[ty][tx]) * dtidyi + (nw - wprev) * dti * m.dzf_inv[k];
+      wprev = nw;
+    }
     if (more) commit(sm);
     __syncthreads();
     const int t = sm;
@@ -705,28 +992,96 @@ inline int forced_mask(const pll_context* c) {
   return (c->is_forced[0] ? 1 : 0) | (c->is_forced[1] ? 2 : 0) | (c->is_forced[2] ? 4 : 0);
 }
 
+// Per-plane coefficients of the FP32-mixed operator for stage `stage`
+// (S = dt or dt/2), computed in FP64 and rounded once.
+RkUniform rk_uniform(const pll_context* c, int stage) {
+  const double S = stage == 1 ? c->dt : 0.5 * c->dt;
+  return RkUniform{(float)(S * 0.25 * c->dx_inv), (float)(S * 0.25 * c->dy_inv),
+                   (float)(S * c->re_inv * c->dx_inv * c->dx_inv),
+                   (float)(S * c->re_inv * c->dy_inv * c->dy_inv),
+                   (float)(S * 0.5 * c->u_crf * c->dx_inv)};
+}
+
+int rk_coef_setup(pll_context* c) {
+  const int nz = c->g.nz;
+  std::vector<RkCoef> h(2 * (size_t)(nz + 2));
+  for (int stage = 1; stage <= 2; ++stage) {
+    const double S = stage == 1 ? c->dt : 0.5 * c->dt, rei = c->re_inv;
+    for (int k = 1; k <= nz; ++k) {
+      const double* f = c->dzf.data();
+      const double* fi = c->dzf_inv.data();
+      const double* ci = c->dzc_inv.data();
+      RkCoef& q = h[(stage - 1) * (size_t)(nz + 2) + k];
+      q.a1 = (float)(f[k + 1] * ci[k]);
+      q.a2 = (float)(f[k] * ci[k]);
+      q.b1 = (float)(f[k - 1] * ci[k - 1]);
+      q.b2 = (float)(f[k] * ci[k - 1]);
+      q.zuv = (float)(S * 0.25 * fi[k]);
+      q.cp = (float)(S * rei * ci[k] * fi[k]);
+      q.cm = (float)(S * rei * ci[k - 1] * fi[k]);
+      q.zw = (float)(S * 0.25 * ci[k]);
+      q.dp = (float)(S * rei * fi[k + 1] * ci[k]);
+      q.dm = (float)(S * rei * fi[k] * ci[k]);
+      q.pad0 = q.pad1 = 0.f;
+    }
+  }
+  PLL_HIP_TRY(hipMalloc(&c->d_rkcoef, h.size() * sizeof(RkCoef)));
+  PLL_HIP_TRY(hipMemcpy(c->d_rkcoef, h.data(), h.size() * sizeof(RkCoef), hipMemcpyHostToDevice));
+  return PLL_OK;
+}
+
 template <typename T>
 int rk_t(pll_context* c, int stage, int fold) {
   const TileGrid t = tile_grid(c);
   const MeshT<T> m = mesh_of<T>(c);
   const T bc_u = (T)(0.0 - c->u_crf);  // src/mod_updateBound.cc:80
   const double* force = c->d_scalar + 3;
+  const RkCoef* coef = static_cast<const RkCoef*>(c->d_rkcoef) + (stage - 1) * (c->g.nz + 2);
+  const RkUniform cu = rk_uniform(c, stage);
+  const int crf = c->u_crf != 0.0;
   if (stage == 1) {
     TimerScope ts(c, K_RK1);
     hipLaunchKernelGGL((k_rk<T, 1>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                        F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
                        F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
-                       bc_u, force, forced_mask(c));
+                       bc_u, force, forced_mask(c), coef, cu, crf);
   } else {
     TimerScope ts(c, K_RK2);
     hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                        F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
                        F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, 0, fold,
-                       bc_u, force, forced_mask(c));
+                       bc_u, force, forced_mask(c), coef, cu, crf);
   }
   PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+// RK stage 2 + calcRHS into the spare velocity buffers, then swap them in.
+template <typename T>
+int rk2_rhs_t(pll_context* c, int defer) {
+  const TileGrid t = tile_grid(c);
+  const MeshT<T> m = mesh_of<T>(c);
+  const T bc_u = (T)(0.0 - c->u_crf);  // src/mod_updateBound.cc:80
+  const RkCoef* coef = static_cast<const RkCoef*>(c->d_rkcoef) + (c->g.nz + 2);
+  const RkUniform cu = rk_uniform(c, 2);
+  const double dti = 1.0 / c->dt;  // src/mod_calcRHS.cc:11-13
+  const double dtidxi = dti * c->dx_inv;
+  const double dtidyi = dti * c->dy_inv;
+  {
+    TimerScope ts(c, K_RK2);
+    hipLaunchKernelGGL((k_rk2_rhs<T>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
+                       F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
+                       F<T>(c, PLL_V), F<T>(c, PLL_W), static_cast<T*>(c->fld_alt[0]),
+                       static_cast<T*>(c->fld_alt[1]), static_cast<T*>(c->fld_alt[2]),
+                       F<T>(c, PLL_P), m, (T)c->dx_inv, (T)c->dy_inv, (T)c->re_inv, (T)c->dt,
+                       (T)c->u_crf, t.kchunk, t.ntx, t.nty, defer, bc_u, c->d_scalar + 3,
+                       forced_mask(c), coef, cu, (int)(c->u_crf != 0.0), (T)dti, (T)dtidxi,
+                       (T)dtidyi);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  for (int d = 0; d < 3; ++d) std::swap(c->fld[PLL_U + d], c->fld_alt[d]);
   return PLL_OK;
 }
 
@@ -935,10 +1290,15 @@ int nan_t(pll_context* c, int field, int* passed) {
 #define PLL_DISPATCH(c, call_fp64, call_fp32) \
   ((c)->prec == PLL_FP64 ? (call_fp64) : (call_fp32))
 
+int rk_setup(pll_context* c) { return c->prec == PLL_FP64 ? PLL_OK : rk_coef_setup(c); }
+
 int launch_rk(pll_context* c, int stage, int fold) {
   return PLL_DISPATCH(c, rk_t<double>(c, stage, fold), rk_t<float>(c, stage, fold));
 }
 int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
+int launch_rk2_rhs(pll_context* c, int defer) {
+  return PLL_DISPATCH(c, rk2_rhs_t<double>(c, defer), rk2_rhs_t<float>(c, defer));
+}
 int launch_correct_vel(pll_context* c, bool with_mean) {
   return PLL_DISPATCH(c, correct_t<double>(c, with_mean), correct_t<float>(c, with_mean));
 }

@@ -140,8 +140,10 @@ void release(pll_context* c) {
     if (e) (void)hipEventDestroy(e);
   for (void*& p : c->fld)
     if (p) (void)hipFree(p);
+  for (void*& p : c->fld_alt)
+    if (p) (void)hipFree(p);
   void* bufs[] = {c->spec,   c->d_mesh, c->d_mesh64, c->d_a,    c->d_c,       c->d_piv,
-                  c->d_piv32, c->d_ac32, c->d_lowk,  c->d_tw_x, c->d_tw_y,    c->d_rtw_x, c->d_twl_x, c->d_twl_y,
+                  c->d_piv32, c->d_ac32, c->d_lowk,  c->d_tw_x, c->d_tw_y,    c->d_rtw_x, c->d_twl_x, c->d_twl_y, c->d_rkcoef,
                   c->d_partial, c->d_scalar, c->d_red};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -246,6 +248,12 @@ int create_impl(const pll_config* cfg, const double* dzf_global, pll_context** o
     e = hipMemsetAsync(c->fld[f], 0, c->nfield * c->esz, c->stream);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMemsetAsync(field)"));
   }
+  for (int d = 0; d < 3; ++d) {
+    e = hipMalloc(&c->fld_alt[d], c->nfield * c->esz);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(spare field)"));
+    e = hipMemsetAsync(c->fld_alt[d], 0, c->nfield * c->esz, c->stream);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMemsetAsync(spare field)"));
+  }
   e = hipMalloc(&c->spec, c->nspec * c->esz);
   if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(spec)"));
   rc = c->prec == PLL_FP64 ? upload_mesh<double>(c) : upload_mesh<float>(c);
@@ -265,6 +273,7 @@ int create_impl(const pll_config* cfg, const double* dzf_global, pll_context** o
   if ((e = hipMalloc(&c->d_red, c->nred)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(red)"));
   if ((rc = fft_setup(c))) return bail(rc);
   if ((rc = trid_setup(c))) return bail(rc);
+  if ((rc = rk_setup(c))) return bail(rc);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
     return bail(hip_fail(e, "hipStreamSynchronize"));
   *out = c;
@@ -290,10 +299,19 @@ int step_impl(pll_context* c, bool first, bool last) {
   const int fold = (fused && !first) ? 1 : 0;
   int rc;
   if ((rc = launch_rk(c, 1, fold))) return rc;    // src/main.cc:117
-  if ((rc = launch_bc_vel(c, 1))) return rc;      // :121
-  if ((rc = launch_rk(c, 2, fold))) return rc;    // :125
-  if ((rc = launch_bc_vel(c, 0))) return rc;      // :129
-  if ((rc = launch_calc_rhs(c))) return rc;       // :133
+  if (!fused || c->prec != PLL_FP64) {
+    // FP32-mixed: separate kernels are faster (the fused one is latency-bound there)
+    if ((rc = launch_bc_vel(c, 1))) return rc;    // :121
+    if ((rc = launch_rk(c, 2, fold))) return rc;  // :125
+    if ((rc = launch_bc_vel(c, 0))) return rc;    // :129
+    if ((rc = launch_calc_rhs(c))) return rc;     // :133
+  } else {
+    // :121-133 in one kernel: the u1 boundary is rebuilt on load, the middle
+    // boundary update only matters for calcRHS (computed in the same kernel)
+    // and for w(nz) = 0 (written there); the last step refreshes u1's halos.
+    if ((rc = launch_rk2_rhs(c, fold))) return rc;
+    if (last && (rc = launch_bc_vel(c, 1))) return rc;
+  }
   if ((rc = poisson_impl(c))) return rc;          // :137
   if ((rc = launch_bc_p(c))) return rc;           // :141
   if (!fused) {

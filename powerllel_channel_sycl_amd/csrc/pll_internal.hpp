@@ -78,6 +78,7 @@ struct pll_context {
   hipStream_t stream = nullptr;
 
   void* fld[PLL_NFIELD] = {};
+  void* fld_alt[3] = {};  // spare u, v, w: the fused stage 2 writes here, then the roles swap
   void* spec = nullptr;
 
   // host mesh in FP64 (index 0..nz+1)
@@ -87,6 +88,7 @@ struct pll_context {
   int is_forced[3];
 
   void* d_mesh = nullptr;      // 4*(nz+2) in field precision: dzf, dzf_inv, dzc_inv, dzflzi
+  void* d_rkcoef = nullptr;    // FP32-mixed stencil coefficients, 2 stages x (nz+2)
   double* d_mesh64 = nullptr;  // same, FP64
 
   // tridiagonal solve: a, c (nz), pre-factored pivot table [k][cy][cx]
@@ -162,7 +164,11 @@ struct TimerScope {
 // fold: stage 1 applies the previous step's deferred force and rebuilds the
 // halos on load; stage 2 adds the deferred force to its point value of u.
 int launch_rk(pll_context* c, int stage, int fold);
+int rk_setup(pll_context* c);  // per-plane coefficients of the FP32-mixed stencil
 int launch_calc_rhs(pll_context* c);
+// RK stage 2 fused with calcRHS (step sequence only): writes the new u, v, w to
+// the spare buffers and swaps them in; defer adds the previous step's force.
+int launch_rk2_rhs(pll_context* c, int defer);
 // with_mean: also reduce the bulk mean of the corrected u and store the force
 // of u (vel_force[0] - mean) on the device.
 int launch_correct_vel(pll_context* c, bool with_mean);
