@@ -176,6 +176,26 @@ int pll_calc_max_cfl(pll_context *ctx, pll_value_index *out);
 /* checkNaN (src/mod_utils.cc:77-106): *passed = 1 when no NaN in the whole array. */
 int pll_check_nan(pll_context *ctx, int field, int *passed);
 
+/* ---- decomp_2d ------------------------------------------------------------------ */
+/* The 2D pencil decomposition for p_row x p_col ranks whose pencils all live
+ * on this GPU (decomp_2d_init + decomp_info_init, src/decomp_2d.cc:41-207).
+ * Rank r has coordinates (r / p_col, r % p_col); every pencil keeps x fastest;
+ * device buffers hold all ranks' pencils of one orientation in rank order. */
+typedef struct pll_decomp pll_decomp;
+int pll_decomp_create(int nx, int ny, int nz, int p_row, int p_col, pll_decomp **out);
+int pll_decomp_destroy(pll_decomp *d);
+/* pencil 0 = x, 1 = y, 2 = z: 1-based start, size and element offset of rank's block
+ * (decomp_info xst/xsz, yst/ysz, zst/zsz, src/decomp_2d.hh:35-37). */
+int pll_decomp_pencil(const pll_decomp *d, int pencil, int rank, int st[3], int sz[3], long *offset);
+int pll_decomp_total(const pll_decomp *d, int pencil, long *total);
+/* transpose_{x_to_y,y_to_x,y_to_z,z_to_y}_real (src/decomp_2d.cc:522-589) on
+ * device buffers of 8-byte (or 4-byte) values: a bit-exact permutation.
+ * Synchronous on return. */
+int pll_transpose_x_to_y_real(pll_decomp *d, const void *src, void *dst, int elem_bytes);
+int pll_transpose_y_to_x_real(pll_decomp *d, const void *src, void *dst, int elem_bytes);
+int pll_transpose_y_to_z_real(pll_decomp *d, const void *src, void *dst, int elem_bytes);
+int pll_transpose_z_to_y_real(pll_decomp *d, const void *src, void *dst, int elem_bytes);
+
 /* ---- main loop ------------------------------------------------------------------ */
 /* nsteps repetitions of src/main.cc:116-154: RK1, BCvel(u1), RK2, BCvel(u),
  * calcRHS, Poisson, BCp, correctVel, forceVel, BCvel(u).  Asynchronous. */

@@ -82,6 +82,14 @@ def lib():
     L.orc_trid_coeff.argtypes = [C.c_int, _D, _D, _D, _D]
     L.orc_get_mesh.argtypes = [vp] + [_D] * 6
     L.orc_get_trid.argtypes = [vp] + [_D] * 5
+    L.orc_decomp_create.restype = vp
+    L.orc_decomp_create.argtypes = [C.c_int] * 5
+    L.orc_decomp_destroy.argtypes = [vp]
+    L.orc_decomp_pencil.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_long)]
+    L.orc_decomp_total.restype = C.c_long
+    L.orc_decomp_total.argtypes = [vp, C.c_int]
+    L.orc_transpose.argtypes = [vp, C.c_int, _D, _D]
     _lib = L
     return L
 
@@ -246,3 +254,34 @@ def trid_coeff(dzf0):
 
 def noise_uniform(seed, offset):
     return lib().orc_noise_uniform(seed, offset)
+
+
+TRANSPOSES = {"x_to_y": 0, "y_to_x": 1, "y_to_z": 2, "z_to_y": 3}
+
+
+class OracleDecomp:
+    """decomp_2d restated with mem_split -> Alltoallv -> mem_merge over virtual ranks."""
+
+    def __init__(self, nx, ny, nz, p_row, p_col):
+        self._h = lib().orc_decomp_create(nx, ny, nz, p_row, p_col)
+        self.nranks = p_row * p_col
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_decomp_destroy(h)
+            self._h = None
+
+    def pencil(self, pencil, rank):
+        st, sz, off = (C.c_int * 3)(), (C.c_int * 3)(), C.c_long()
+        lib().orc_decomp_pencil(self._h, pencil, rank, st, sz, C.byref(off))
+        return tuple(st), tuple(sz), off.value
+
+    def total(self, pencil):
+        return lib().orc_decomp_total(self._h, pencil)
+
+    def transpose(self, kind, src):
+        dst_pencil = {"x_to_y": 1, "y_to_x": 0, "y_to_z": 2, "z_to_y": 1}[kind]
+        out = np.zeros(self.total(dst_pencil))
+        lib().orc_transpose(self._h, TRANSPOSES[kind], np.ascontiguousarray(src, dtype=np.float64), out)
+        return out

@@ -18,7 +18,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Channel", "PllError", "load_library", "mesh_by_func", "eigen_values", "FIELDS",
+__all__ = ["Channel", "Decomp", "PllError", "load_library", "mesh_by_func", "eigen_values", "FIELDS",
            "PRECISIONS", "PROFILES", "LIB_PATH", "HEADER_PATH"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -94,6 +94,15 @@ _SIGNATURES = {
     "pll_calc_max_div": (C.c_int, [C.c_void_p, C.POINTER(PllValueIndex)]),
     "pll_calc_max_cfl": (C.c_int, [C.c_void_p, C.POINTER(PllValueIndex)]),
     "pll_check_nan": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "pll_decomp_create": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_void_p)]),
+    "pll_decomp_destroy": (C.c_int, [C.c_void_p]),
+    "pll_decomp_pencil": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_long)]),
+    "pll_decomp_total": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_long)]),
+    "pll_transpose_x_to_y_real": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "pll_transpose_y_to_x_real": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "pll_transpose_y_to_z_real": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "pll_transpose_z_to_y_real": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "pll_step": (C.c_int, [C.c_void_p, C.c_int]),
     "pll_synchronize": (C.c_int, [C.c_void_p]),
     "pll_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
@@ -140,6 +149,48 @@ def eigen_values(n):
     out = np.zeros(n)
     _check(load_library().pll_eigen_values(n, out), "pll_eigen_values")
     return out
+
+
+class Decomp:
+    """decomp_2d for p_row x p_col ranks held on one GPU (src/decomp_2d.cc:41-589).
+
+    Pencils: 0 = x, 1 = y, 2 = z.  Device buffers passed to the transposes hold
+    every rank's pencil in rank order (rank = coords[0]*p_col + coords[1])."""
+
+    def __init__(self, nx, ny, nz, p_row, p_col):
+        self._h = C.c_void_p()
+        _check(load_library().pll_decomp_create(nx, ny, nz, p_row, p_col, C.byref(self._h)),
+               "pll_decomp_create")
+        self.nranks = p_row * p_col
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_library().pll_decomp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pencil(self, pencil, rank):
+        """(st[3], sz[3], element offset) of rank's pencil; st is 1-based."""
+        st, sz, off = (C.c_int * 3)(), (C.c_int * 3)(), C.c_long()
+        _check(load_library().pll_decomp_pencil(self._h, pencil, rank, st, sz, C.byref(off)),
+               "pll_decomp_pencil")
+        return tuple(st), tuple(sz), off.value
+
+    def total(self, pencil):
+        t = C.c_long()
+        _check(load_library().pll_decomp_total(self._h, pencil, C.byref(t)), "pll_decomp_total")
+        return t.value
+
+    def transpose(self, kind, src_ptr, dst_ptr, elem_bytes=8):
+        """kind in {'x_to_y', 'y_to_x', 'y_to_z', 'z_to_y'} on device addresses."""
+        fn = getattr(load_library(), f"pll_transpose_{kind}_real")
+        _check(fn(self._h, C.c_void_p(src_ptr), C.c_void_p(dst_ptr), elem_bytes),
+               f"pll_transpose_{kind}_real")
 
 
 class Channel:
