@@ -164,6 +164,11 @@ int pll_eigen_values(int n, double *lambda);
 /* ---- mod_updateBound ---------------------------------------------------------- */
 /* updateBoundVel (src/mod_updateBound.cc:229-251): which = 0 -> (u,v,w), 1 -> (u1,v1,w1). */
 int pll_update_bound_vel(pll_context *ctx, int which);
+/* imposeBCVel (src/mod_updateBound.cc:183-227) at one rank: only the wall
+ * conditions (x and y neighbours are the rank itself, so no periodic copy):
+ * u(0) = 2(0-u_crf) - u(1), v(0) = -v(1), w(0) = 0 and the same at the top
+ * with w(nz) = w(nz+1) = 0, over the whole x/y extent including halo cells. */
+int pll_impose_bc_vel(pll_context *ctx, int which);
 /* updateBoundP (src/mod_updateBound.cc:253-296). */
 int pll_update_bound_p(pll_context *ctx);
 
@@ -175,6 +180,21 @@ int pll_calc_max_div(pll_context *ctx, pll_value_index *out);
 int pll_calc_max_cfl(pll_context *ctx, pll_value_index *out);
 /* checkNaN (src/mod_utils.cc:77-106): *passed = 1 when no NaN in the whole array. */
 int pll_check_nan(pll_context *ctx, int field, int *passed);
+
+/* ---- mod_statistics ------------------------------------------------------------ */
+/* The per-point sums of calcStat (src/mod_statistics.cc:30-106), kept on the
+ * device in FP64.  Quantity index: 0 u, 1 v, 2 w, 3 u2, 4 v2, 5 w2, 6 uv,
+ * 7 uw, 8 vw, 9 p, 10 p2 (src/mod_dataIO.cc:39-63), from cell-centred
+ * uc = (u(i-1)+u(i))/2 + u_crf, vc = (v(j-1)+v(j))/2, wc = (w(k-1)+w(k))/2.
+ * Arrays are interior-only nz*ny*nx, x fastest (the checkpoint layout). */
+/* Select quantities (which[t] != 0), allocate their sums and zero them. */
+int pll_stat_enable(pll_context *ctx, const int which[11]);
+/* Add one sample of the current u, v, w, p to every selected sum. */
+int pll_stat_accumulate(pll_context *ctx);
+int pll_stat_download(pll_context *ctx, int var, double *host);
+int pll_stat_upload(pll_context *ctx, int var, const double *host);
+/* host[k-1] = sum over plane k of the sum array of var (k = 1..nz). */
+int pll_stat_plane_sums(pll_context *ctx, int var, double *host);
 
 /* ---- decomp_2d ------------------------------------------------------------------ */
 /* The 2D pencil decomposition for p_row x p_col ranks whose pencils all live

@@ -56,6 +56,7 @@ def lib():
     L.orc_correct_vel.argtypes = [vp, _D, _D, _D, _D]
     L.orc_update_bound_vel.argtypes = [vp, _D, _D, _D]
     L.orc_update_bound_p.argtypes = [vp, _D]
+    L.orc_impose_bc_vel.argtypes = [vp, _D, _D, _D]
     L.orc_mean.restype = C.c_double
     L.orc_mean.argtypes = [vp, C.c_int, C.c_int, _D]
     L.orc_force_vel.argtypes = [vp, _D, _D, _D, _D]
@@ -161,6 +162,9 @@ class Oracle:
 
     def update_bound_vel(self, u, v, w):
         lib().orc_update_bound_vel(self._h, u, v, w)
+
+    def impose_bc_vel(self, u, v, w):
+        lib().orc_impose_bc_vel(self._h, u, v, w)
 
     def update_bound_p(self, p):
         lib().orc_update_bound_p(self._h, p)

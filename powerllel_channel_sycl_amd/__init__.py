@@ -89,11 +89,17 @@ _SIGNATURES = {
     "pll_fft_bwd_xpen": (C.c_int, [C.c_void_p]),
     "pll_eigen_values": (C.c_int, [C.c_int, _D]),
     "pll_update_bound_vel": (C.c_int, [C.c_void_p, C.c_int]),
+    "pll_impose_bc_vel": (C.c_int, [C.c_void_p, C.c_int]),
     "pll_update_bound_p": (C.c_int, [C.c_void_p]),
     "pll_mean": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "pll_calc_max_div": (C.c_int, [C.c_void_p, C.POINTER(PllValueIndex)]),
     "pll_calc_max_cfl": (C.c_int, [C.c_void_p, C.POINTER(PllValueIndex)]),
     "pll_check_nan": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "pll_stat_enable": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "pll_stat_accumulate": (C.c_int, [C.c_void_p]),
+    "pll_stat_download": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "pll_stat_upload": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "pll_stat_plane_sums": (C.c_int, [C.c_void_p, C.c_int, _D]),
     "pll_decomp_create": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_void_p)]),
     "pll_decomp_destroy": (C.c_int, [C.c_void_p]),
     "pll_decomp_pencil": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
@@ -323,8 +329,35 @@ class Channel:
     def updateBoundVel(self, which=0):
         self._call("pll_update_bound_vel", int(which))
 
+    def imposeBCVel(self, which=0):
+        """Wall conditions only (src/mod_updateBound.cc:183-227)."""
+        self._call("pll_impose_bc_vel", int(which))
+
     def updateBoundP(self):
         self._call("pll_update_bound_p")
+
+    # ---- mod_statistics ---------------------------------------------------------------
+    STAT_NAMES = ("u", "v", "w", "u2", "v2", "w2", "uv", "uw", "vw", "p", "p2")
+
+    def stat_enable(self, which=(1,) * 11):
+        arr = (C.c_int * 11)(*[int(bool(x)) for x in which])
+        self._call("pll_stat_enable", arr)
+
+    def stat_accumulate(self):
+        self._call("pll_stat_accumulate")
+
+    def stat_download(self, var):
+        out = np.zeros((self.nz, self.ny, self.nx))
+        self._call("pll_stat_download", int(var), out)
+        return out
+
+    def stat_upload(self, var, arr):
+        self._call("pll_stat_upload", int(var), np.ascontiguousarray(arr, dtype=np.float64))
+
+    def stat_plane_sums(self, var):
+        out = np.zeros(self.nz)
+        self._call("pll_stat_plane_sums", int(var), out)
+        return out
 
     # ---- mod_utils -------------------------------------------------------------------
     def Mean(self, field="u", nx_global=None, ny_global=None):

@@ -24,12 +24,14 @@ struct pll_decomp {
   // per rank and pencil: start (1-based), size, element offset
   std::vector<int> st, sz;  // [(r*3 + pencil)*3 + axis]
   std::vector<long> off;    // [r*3 + pencil]
+  std::vector<long> rows;   // [pencil*(P+1) + r]: first x-row of rank r in the pencil
   long total[3];
   std::vector<int> meta;    // for each rank and pencil st[3], sz[3]
   std::vector<int> dists;   // y1st, x1st, z2st, y2st one after another
   int dist_at[4];           // where each starts in dists, by transpose kind
   int* d_meta = nullptr;    // device copies, made by the first transpose
   long* d_off = nullptr;
+  long* d_rows = nullptr;
   int* d_dists = nullptr;
 };
 
@@ -75,34 +77,51 @@ struct XposeArgs {
   int ps, pd;       // source and destination pencil
   int split_axis;   // axis whose distribution picks the source rank
   int dist_len;     // length of that distribution
-  long total;       // destination element count
+  long rows;        // destination row count (rows of x)
 };
 
-// One thread per destination value.  meta: [(r*3+pencil)*6 + {st0,st1,st2,sz0,sz1,sz2}].
+// One wave per destination row (fixed y, z of one destination rank), lanes
+// along x: the rank and row bookkeeping is done once per row, and both the
+// loads and the stores of a wave are contiguous runs.  rows_off[r] is the
+// first row of rank r in the destination; meta: [(r*3+pencil)*6 + {st, sz}].
 template <typename E>
 __global__ __launch_bounds__(256) void k_transpose(XposeArgs a, const int* __restrict__ meta,
                                                    const long* __restrict__ off,
+                                                   const long* __restrict__ rows_off,
                                                    const int* __restrict__ dist_st,
                                                    const E* __restrict__ src, E* __restrict__ dst) {
   const int P = a.p_row * a.p_col;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < a.total;
-       t += (long)gridDim.x * blockDim.x) {
-    // destination rank by offset (ranks are few; linear scan)
+  const int lane = threadIdx.x & 63;
+  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+  for (long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); row < a.rows;
+       row += nwaves) {
     int r = 0;
-    while (r + 1 < P && off[(r + 1) * 3 + a.pd] <= t) ++r;
+    while (r + 1 < P && rows_off[r + 1] <= row) ++r;
     const int* dm = meta + (r * 3 + a.pd) * 6;
-    const long l = t - off[r * 3 + a.pd];
-    const int li = (int)(l % dm[3]);
-    const int lj = (int)((l / dm[3]) % dm[4]);
-    const int lk = (int)(l / ((long)dm[3] * dm[4]));
-    const int g[3] = {dm[0] + li, dm[1] + lj, dm[2] + lk};
-    // the source rank shares the destination's coordinate outside the sub-communicator
+    const long lr = row - rows_off[r];
+    const int lj = (int)(lr % dm[4]);
+    const int lk = (int)(lr / dm[4]);
+    const int gj = dm[1] + lj, gk = dm[2] + lk;
+    E* drow = dst + off[r * 3 + a.pd] + lr * dm[3];
     const int c0 = r / a.p_col, c1 = r % a.p_col;
-    const int m = owner(dist_st, a.dist_len, g[a.split_axis]);
-    const int rs = a.col ? (m * a.p_col + c1) : (c0 * a.p_col + m);
-    const int* sm = meta + (rs * 3 + a.ps) * 6;
-    const long ls = (long)(g[0] - sm[0]) + (long)sm[3] * ((g[1] - sm[1]) + (long)sm[4] * (g[2] - sm[2]));
-    dst[t] = src[off[rs * 3 + a.ps] + ls];
+    if (a.split_axis == 0) {
+      // y -> x: the x row gathers from p_row source pencils, one run each
+      for (int li = lane; li < dm[3]; li += 64) {
+        const int gi = dm[0] + li;
+        const int m = owner(dist_st, a.dist_len, gi);
+        const int rs = m * a.p_col + c1;
+        const int* sm = meta + (rs * 3 + a.ps) * 6;
+        drow[li] = src[off[rs * 3 + a.ps] + (gi - sm[0]) +
+                       (long)sm[3] * ((gj - sm[1]) + (long)sm[4] * (gk - sm[2]))];
+      }
+    } else {
+      const int m = owner(dist_st, a.dist_len, a.split_axis == 1 ? gj : gk);
+      const int rs = a.col ? (m * a.p_col + c1) : (c0 * a.p_col + m);
+      const int* sm = meta + (rs * 3 + a.ps) * 6;
+      const E* srow = src + off[rs * 3 + a.ps] + (dm[0] - sm[0]) +
+                      (long)sm[3] * ((gj - sm[1]) + (long)sm[4] * (gk - sm[2]));
+      for (int li = lane; li < dm[3]; li += 64) drow[li] = srow[li];
+    }
   }
 }
 
@@ -159,6 +178,11 @@ int pll_decomp_create(int nx, int ny, int nz, int p_row, int p_col, pll_decomp**
     }
   }
   d->meta = meta;
+  d->rows.assign(3 * (P + 1), 0);
+  for (int p = 0; p < 3; ++p)
+    for (int r = 0; r < P; ++r)
+      d->rows[p * (P + 1) + r + 1] =
+          d->rows[p * (P + 1) + r] + (long)d->sz[(r * 3 + p) * 3 + 1] * d->sz[(r * 3 + p) * 3 + 2];
   const std::vector<int>* by_kind[4] = {&d->y1st, &d->x1st, &d->z2st, &d->y2st};
   for (int k = 0; k < 4; ++k) {
     d->dist_at[k] = (int)d->dists.size();
@@ -173,6 +197,7 @@ int pll_decomp_destroy(pll_decomp* d) {
   if (d->d_meta) (void)hipFree(d->d_meta);
   if (d->d_off) (void)hipFree(d->d_off);
   if (d->d_dists) (void)hipFree(d->d_dists);
+  if (d->d_rows) (void)hipFree(d->d_rows);
   delete d;
   return PLL_OK;
 }
@@ -212,11 +237,15 @@ static int transpose(pll_decomp* d, int kind, const void* src, void* dst, int el
     default:  // z -> y within comm_row: source rank by the y2 owner of global y
       a.ps = 2; a.pd = 1; a.col = 0; a.split_axis = 1; a.dist_len = d->p_col; break;
   }
-  a.total = d->total[a.pd];
+  const int P = d->p_row * d->p_col;
+  a.rows = d->rows[a.pd * (P + 1) + P];
   if (!d->d_meta) {
     PLL_HIP_TRY(hipMalloc(&d->d_meta, d->meta.size() * sizeof(int)));
     PLL_HIP_TRY(hipMalloc(&d->d_off, d->off.size() * sizeof(long)));
     PLL_HIP_TRY(hipMalloc(&d->d_dists, d->dists.size() * sizeof(int)));
+    PLL_HIP_TRY(hipMalloc(&d->d_rows, d->rows.size() * sizeof(long)));
+    PLL_HIP_TRY(hipMemcpy(d->d_rows, d->rows.data(), d->rows.size() * sizeof(long),
+                          hipMemcpyHostToDevice));
     PLL_HIP_TRY(hipMemcpy(d->d_meta, d->meta.data(), d->meta.size() * sizeof(int),
                           hipMemcpyHostToDevice));
     PLL_HIP_TRY(hipMemcpy(d->d_off, d->off.data(), d->off.size() * sizeof(long),
@@ -224,15 +253,16 @@ static int transpose(pll_decomp* d, int kind, const void* src, void* dst, int el
     PLL_HIP_TRY(hipMemcpy(d->d_dists, d->dists.data(), d->dists.size() * sizeof(int),
                           hipMemcpyHostToDevice));
   }
-  if (a.total == 0) return PLL_OK;
+  if (a.rows == 0) return PLL_OK;
   const int* dist = d->d_dists + d->dist_at[kind];
-  const unsigned blocks = (unsigned)std::min<long>((a.total + 255) / 256, 65536L);
+  const long* rows_off = d->d_rows + a.pd * (P + 1);
+  const unsigned blocks = (unsigned)std::min<long>((a.rows + 3) / 4, 65536L);
   if (elem_bytes == 8)
     hipLaunchKernelGGL((k_transpose<double>), dim3(blocks), dim3(256), 0, 0, a, d->d_meta,
-                       d->d_off, dist, static_cast<const double*>(src), static_cast<double*>(dst));
+                       d->d_off, rows_off, dist, static_cast<const double*>(src), static_cast<double*>(dst));
   else
     hipLaunchKernelGGL((k_transpose<float>), dim3(blocks), dim3(256), 0, 0, a, d->d_meta,
-                       d->d_off, dist, static_cast<const float*>(src), static_cast<float*>(dst));
+                       d->d_off, rows_off, dist, static_cast<const float*>(src), static_cast<float*>(dst));
   PLL_HIP_TRY(hipGetLastError());
   PLL_HIP_TRY(hipDeviceSynchronize());
   return PLL_OK;

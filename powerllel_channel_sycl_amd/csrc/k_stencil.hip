@@ -721,10 +721,10 @@ __device__ __forceinline__ void perimeter(int q, int nx, int ny, int& i, int& j)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_bc_vel(Geom g, T* __restrict__ u, T* __restrict__ v,
-                                                T* __restrict__ w, T bc_u) {
+                                                T* __restrict__ w, T bc_u, int walls_only) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int nper = 2 * (g.nx + 2) + 2 * g.ny;
-  const long nside = (long)nper * g.nz;
+  const long nside = walls_only ? 0 : (long)nper * g.nz;
   const long nplane = (long)(g.nx + 2) * (g.ny + 2);
   const T zero = T(0), two = T(2);
   if (t < nside) {
@@ -740,7 +740,9 @@ __global__ __launch_bounds__(256) void k_bc_vel(Geom g, T* __restrict__ u, T* __
     const long q = t - nside;
     const int i = (int)(q % (g.nx + 2));
     const int j = (int)(q / (g.nx + 2));
-    const int wi = wrap(i, g.nx), wj = wrap(j, g.ny);
+    // after the exchange the wall rows read the periodic image; alone
+    // (imposeBCVel) they read the halo cells as they stand
+    const int wi = walls_only ? i : wrap(i, g.nx), wj = walls_only ? j : wrap(j, g.ny);
     u[g.idx(i, j, 0)] = two * bc_u - u[g.idx(wi, wj, 1)];
     u[g.idx(i, j, g.nz + 1)] = two * bc_u - u[g.idx(wi, wj, g.nz)];
     v[g.idx(i, j, 0)] = two * zero - v[g.idx(wi, wj, 1)];
@@ -1187,13 +1189,14 @@ inline long bc_threads(const pll_context* c) {
 }
 
 template <typename T>
-int bc_vel_t(pll_context* c, int which) {
-  const long n = bc_threads(c);
+int bc_vel_t(pll_context* c, int which, int walls_only) {
+  const long n = walls_only ? (long)(c->g.nx + 2) * (c->g.ny + 2) : bc_threads(c);
   const int base = which ? PLL_U1 : PLL_U;
   const double bc_u = 0.0 - c->u_crf;  // src/mod_updateBound.cc:80
   TimerScope ts(c, K_BC_VEL);
   hipLaunchKernelGGL((k_bc_vel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     c->g, F<T>(c, base), F<T>(c, base + 1), F<T>(c, base + 2), (T)bc_u);
+                     c->g, F<T>(c, base), F<T>(c, base + 1), F<T>(c, base + 2), (T)bc_u,
+                     walls_only);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
@@ -1310,8 +1313,9 @@ int launch_mean(pll_context* c, int field, double nxg, double nyg, double* d_out
                       mean_t<float>(c, field, nxg, nyg, d_out, 0.0, nullptr));
 }
 int launch_force(pll_context* c) { return PLL_DISPATCH(c, force_t<double>(c), force_t<float>(c)); }
-int launch_bc_vel(pll_context* c, int which) {
-  return PLL_DISPATCH(c, bc_vel_t<double>(c, which), bc_vel_t<float>(c, which));
+int launch_bc_vel(pll_context* c, int which, int walls_only) {
+  return PLL_DISPATCH(c, bc_vel_t<double>(c, which, walls_only),
+                      bc_vel_t<float>(c, which, walls_only));
 }
 int launch_bc_p(pll_context* c) { return PLL_DISPATCH(c, bc_p_t<double>(c), bc_p_t<float>(c)); }
 int launch_add_scalar(pll_context* c, int field, double value) {

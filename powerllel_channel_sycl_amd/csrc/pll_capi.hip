@@ -142,6 +142,8 @@ void release(pll_context* c) {
     if (p) (void)hipFree(p);
   for (void*& p : c->fld_alt)
     if (p) (void)hipFree(p);
+  for (double*& p : c->d_stat)
+    if (p) (void)hipFree(p);
   void* bufs[] = {c->spec,   c->d_mesh, c->d_mesh64, c->d_a,    c->d_c,       c->d_piv,
                   c->d_piv32, c->d_ac32, c->d_lowk,  c->d_tw_x, c->d_tw_y,    c->d_rtw_x, c->d_twl_x, c->d_twl_y, c->d_rkcoef,
                   c->d_partial, c->d_scalar, c->d_red};
@@ -545,6 +547,12 @@ int pll_update_bound_vel(pll_context* c, int which) {
   if (!c) return fail(PLL_ERR_ARG, "null context");
   if (which != 0 && which != 1) return fail(PLL_ERR_ARG, "which must be 0 (u,v,w) or 1 (u1,v1,w1)");
   return launch_bc_vel(c, which);
+}
+
+int pll_impose_bc_vel(pll_context* c, int which) {
+  if (!c) return fail(PLL_ERR_ARG, "null context");
+  if (which != 0 && which != 1) return fail(PLL_ERR_ARG, "which must be 0 (u,v,w) or 1 (u1,v1,w1)");
+  return launch_bc_vel(c, which, 1);
 }
 
 int pll_update_bound_p(pll_context* c) {

@@ -120,6 +120,10 @@ struct pll_context {
   void* d_red = nullptr;       // check-reduction scratch
   long nred = 0;
 
+  // statistics: FP64 per-point sums of the 11 sampled quantities (interior,
+  // nz*ny*nx, x fastest); null when not selected
+  double* d_stat[11] = {};
+
   // timers
   bool timing = false;
   std::vector<pll::PendingTimer> pending;
@@ -175,7 +179,7 @@ int launch_correct_vel(pll_context* c, bool with_mean);
 int launch_add_force_u(pll_context* c);
 int launch_mean(pll_context* c, int field, double nxg, double nyg, double* d_out);
 int launch_force(pll_context* c);
-int launch_bc_vel(pll_context* c, int which);
+int launch_bc_vel(pll_context* c, int which, int walls_only = 0);
 int launch_bc_p(pll_context* c);
 int launch_add_scalar(pll_context* c, int field, double value);
 int launch_init_flow(pll_context* c, const double* d_prof, int with_noise, double noise,
