@@ -36,12 +36,15 @@ DEFAULT_WORKLOAD = "retau550_768x384x384"
 
 # Algorithmic (compulsory) HBM bytes per grid point per launch, FP64; FP32 halves them.
 # RK1 reads u,v,w and writes u1,v1,w1; RK2 reads u1,v1,w1,u,v,w and writes u,v,w;
-# calcRHS reads u,v,w and writes p; each transform and the tridiagonal solve read
-# and write the work array once; correctVel reads p,u,v,w and writes u,v,w; the
-# bulk mean reads u; the force update reads and writes u.
+# calcRHS reads u,v,w and writes p; RK2 fused with calcRHS (the FP64 step) reads
+# u1,v1,w1,u,v,w and writes u,v,w,p; each transform and the tridiagonal solve read
+# and write the work array once; correctVel reads p,u,v,w and writes u,v,w (the
+# bulk mean of u rides along); the separate bulk mean reads u; the force update
+# reads and writes u.
 BYTES_PER_POINT_FP64 = {
-    "rk1": 48, "rk2": 72, "calc_rhs": 32, "fft_x_fwd": 16, "fft_y_fwd": 16, "solve_trid": 16,
-    "fft_y_bwd": 16, "fft_x_bwd": 16, "correct_vel": 56, "mean_partial": 8, "force_add": 16,
+    "rk1": 48, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 80, "fft_x_fwd": 16, "fft_y_fwd": 16,
+    "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "correct_vel": 56, "mean_partial": 8,
+    "force_add": 16,
 }
 STEP_BYTES_FP64 = 312
 

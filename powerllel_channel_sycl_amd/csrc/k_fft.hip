@@ -401,27 +401,52 @@ __device__ __forceinline__ T& real_at(cpx<T>* buf, int e) {
   return reinterpret_cast<T*>(buf)[2 * pad(e >> 1) + (e & 1)];
 }
 
-// execute_fft_fwd_xpen (src/mod_fft.cc:60-88), L = nx/2 points per row.
-template <typename T, int L>
+// The velocity divergence of calcRHS (src/mod_calcRHS.cc:7-33), for a forward
+// x transform that reads it straight from u, v, w instead of from p.
+template <typename T>
+struct RhsSrc {
+  const T* u;
+  const T* v;
+  const T* w;
+  const T* dzf_inv;
+  T dti, dtidxi, dtidyi;
+};
+
+// execute_fft_fwd_xpen (src/mod_fft.cc:60-88), L = nx/2 points per row.  With
+// RHS the rows are calcRHS's output computed on load (nothing is stored in p).
+template <typename T, int L, bool RHS>
 __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xf(Geom g, const T* __restrict__ p,
                                                                T* __restrict__ spec,
                                                                const cpx<T>* __restrict__ twL,
                                                                const cpx<T>* __restrict__ rtw,
-                                                               long nrows) {
+                                                               long nrows, RhsSrc<T> rs) {
   constexpr int TPS = tps<L>(), NB = x_rows<T, L>(), NT = NB * TPS, N = 2 * L;
   __shared__ cpx<T> buf[padded(NB * L)];
   __shared__ long rowbase[NB];
+  __shared__ T rowdz[NB];
   const long row0 = (long)xcd_remap(blockIdx.x, gridDim.x) * NB;
   if (threadIdx.x < NB) {
     const long r = row0 + threadIdx.x;
     const int j = (int)(r % g.ny), k = (int)(r / g.ny);
     rowbase[threadIdx.x] = r < nrows ? g.idx(1, j + 1, k + 1) : -1;
+    if (RHS) rowdz[threadIdx.x] = r < nrows ? rs.dti * rs.dzf_inv[k + 1] : T(0);
   }
   __syncthreads();
   for (int e = threadIdx.x; e < NB * N; e += NT) {
     const int s = e / N, i = e % N;
     const long b = rowbase[s];
-    real_at(buf, s * L * 2 + i) = b >= 0 ? p[b + i] : T(0);
+    T val = T(0);
+    if (b >= 0) {
+      const long c = b + i;
+      if (RHS) {
+        // src/mod_calcRHS.cc:21-28, same association as k_calc_rhs
+        val = (rs.u[c] - rs.u[c - 1]) * rs.dtidxi + (rs.v[c] - rs.v[c - g.sx]) * rs.dtidyi +
+              (rs.w[c] - rs.w[c - g.sxy]) * rowdz[s];
+      } else {
+        val = p[c];
+      }
+    }
+    real_at(buf, s * L * 2 + i) = val;
   }
   __syncthreads();
   const int s = threadIdx.x / TPS, jj = threadIdx.x % TPS;
@@ -697,17 +722,26 @@ int y_generic(pll_context* c, bool fwd) {
 
 // ---- fixed-length kernels ------------------------------------------------------------
 template <typename T, int L>
-int x_fixed(pll_context* c, bool fwd) {
+int x_fixed(pll_context* c, bool fwd, bool rhs = false) {
   constexpr int NB = x_rows<T, L>(), NT = NB * tps<L>();
   const long nrows = (long)c->g.ny * c->g.nz;
   const unsigned blocks = (unsigned)((nrows + NB - 1) / NB);
   const cpx<T>* tw = static_cast<const cpx<T>*>(c->d_twl_x);
   const cpx<T>* rtw = static_cast<const cpx<T>*>(c->d_rtw_x);
-  if (fwd) {
+  if (fwd && rhs) {
+    const double dti = 1.0 / c->dt;  // src/mod_calcRHS.cc:11-13
+    RhsSrc<T> rs{static_cast<const T*>(c->fld[PLL_U]), static_cast<const T*>(c->fld[PLL_V]),
+                 static_cast<const T*>(c->fld[PLL_W]),
+                 static_cast<const T*>(c->d_mesh) + (c->g.nz + 2),  // dzf_inv
+                 (T)dti, (T)(dti * c->dx_inv), (T)(dti * c->dy_inv)};
+    TimerScope ts(c, K_FFT_X_FWD_RHS);
+    hipLaunchKernelGGL((k_xf<T, L, true>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+                       static_cast<const T*>(nullptr), static_cast<T*>(c->spec), tw, rtw, nrows, rs);
+  } else if (fwd) {
     TimerScope ts(c, K_FFT_X_FWD);
-    hipLaunchKernelGGL((k_xf<T, L>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+    hipLaunchKernelGGL((k_xf<T, L, false>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
                        static_cast<const T*>(c->fld[PLL_P]), static_cast<T*>(c->spec), tw, rtw,
-                       nrows);
+                       nrows, RhsSrc<T>{});
   } else {
     const double norm = 1.0 / (c->g.nx * c->g.ny);  // src/mod_fft.cc:57
     TimerScope ts(c, K_FFT_X_BWD);
@@ -820,6 +854,24 @@ int fft_setup(pll_context* c) {
   return c->prec == PLL_FP64 ? setup_fixed<double>(c) : setup_fixed<float>(c);
 }
 int launch_fft_x_fwd(pll_context* c) { return c->prec == PLL_FP64 ? x_t<double>(c, true) : x_t<float>(c, true); }
+
+template <typename T>
+int x_rhs_t(pll_context* c) {
+  switch (c->g.nx / 2) {
+#define PLL_CASE(n) case n: return x_fixed<T, n>(c, true, true);
+    PLL_X_SIZES(PLL_CASE)
+#undef PLL_CASE
+    default: break;
+  }
+  return fail(PLL_ERR_UNSUPPORTED, "no fixed-length x transform for calcRHS fusion");
+}
+
+bool fft_x_rhs_available(const pll_context* c) { return c->x_fixed; }
+
+int launch_fft_x_fwd_rhs(pll_context* c) {
+  if (!c->x_fixed) return fail(PLL_ERR_UNSUPPORTED, "no fixed-length x transform for calcRHS fusion");
+  return c->prec == PLL_FP64 ? x_rhs_t<double>(c) : x_rhs_t<float>(c);
+}
 int launch_fft_x_bwd(pll_context* c) { return c->prec == PLL_FP64 ? x_t<double>(c, false) : x_t<float>(c, false); }
 int launch_fft_y_fwd(pll_context* c) { return c->prec == PLL_FP64 ? y_t<double>(c, true) : y_t<float>(c, true); }
 int launch_fft_y_bwd(pll_context* c) { return c->prec == PLL_FP64 ? y_t<double>(c, false) : y_t<float>(c, false); }

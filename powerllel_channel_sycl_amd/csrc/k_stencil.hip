@@ -1072,7 +1072,7 @@ int rk2_rhs_t(pll_context* c, int defer) {
   const double dtidxi = dti * c->dx_inv;
   const double dtidyi = dti * c->dy_inv;
   {
-    TimerScope ts(c, K_RK2);
+    TimerScope ts(c, K_RK2_RHS);
     hipLaunchKernelGGL((k_rk2_rhs<T>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                        F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
                        F<T>(c, PLL_V), F<T>(c, PLL_W), static_cast<T*>(c->fld_alt[0]),

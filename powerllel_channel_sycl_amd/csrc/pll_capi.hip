@@ -14,7 +14,8 @@ namespace pll {
 const char* const kKernelNames[K_COUNT] = {
     "rk1",         "rk2",       "calc_rhs",   "fft_x_fwd",    "fft_y_fwd", "solve_trid",
     "fft_y_bwd",   "fft_x_bwd", "correct_vel", "mean_partial", "mean_final", "force_add",
-    "bc_vel",      "bc_p",      "check",      "init",         "misc"};
+    "bc_vel",      "bc_p",      "check",      "init",         "misc",        "rk2_rhs",
+    "fft_x_fwd_rhs"};
 
 namespace {
 thread_local std::string g_last_error;
@@ -296,6 +297,8 @@ int poisson_impl(pll_context* c) {
 // force update and the final boundary update of a step (:149-153) are folded
 // into the next step's RK loads (`fold`); the last step of a run applies them
 // explicitly, so every run ends in exactly the reference's state.
+int finish_step(pll_context* c, bool fused, bool last);
+
 int step_impl(pll_context* c, bool first, bool last) {
   const bool fused = !c->is_forced[1] && !c->is_forced[2];
   const int fold = (fused && !first) ? 1 : 0;
@@ -306,6 +309,15 @@ int step_impl(pll_context* c, bool first, bool last) {
     if ((rc = launch_bc_vel(c, 1))) return rc;    // :121
     if ((rc = launch_rk(c, 2, fold))) return rc;  // :125
     if ((rc = launch_bc_vel(c, 0))) return rc;    // :129
+    if (c->prec != PLL_FP64 && fft_x_rhs_available(c)) {
+      // :133-137 with calcRHS computed inside the forward x transform's loads
+      if ((rc = launch_fft_x_fwd_rhs(c))) return rc;
+      if ((rc = launch_fft_y_fwd(c))) return rc;
+      if ((rc = launch_trid(c))) return rc;
+      if ((rc = launch_fft_y_bwd(c))) return rc;
+      if ((rc = launch_fft_x_bwd(c))) return rc;
+      return finish_step(c, fused, last);
+    }
     if ((rc = launch_calc_rhs(c))) return rc;     // :133
   } else {
     // :121-133 in one kernel: the u1 boundary is rebuilt on load, the middle
@@ -315,6 +327,12 @@ int step_impl(pll_context* c, bool first, bool last) {
     if (last && (rc = launch_bc_vel(c, 1))) return rc;
   }
   if ((rc = poisson_impl(c))) return rc;          // :137
+  return finish_step(c, fused, last);
+}
+
+// src/main.cc:141-153 after the pressure solve.
+int finish_step(pll_context* c, bool fused, bool last) {
+  int rc;
   if ((rc = launch_bc_p(c))) return rc;           // :141
   if (!fused) {
     if ((rc = launch_correct_vel(c, false))) return rc;  // :145

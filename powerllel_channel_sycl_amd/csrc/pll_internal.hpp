@@ -56,6 +56,8 @@ enum KernelId {
   K_CHECK,
   K_INIT,
   K_MISC,
+  K_RK2_RHS,        // stage 2 fused with calcRHS
+  K_FFT_X_FWD_RHS,  // calcRHS fused into the forward x transform
   K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
@@ -192,6 +194,9 @@ int launch_check_nan(pll_context* c, int field, int* passed);
 // transforms and the tridiagonal solve (k_fft.hip, k_trid.hip)
 int fft_setup(pll_context* c);
 int launch_fft_x_fwd(pll_context* c);
+// calcRHS fused into the forward x transform (reads u, v, w; p is not written)
+bool fft_x_rhs_available(const pll_context* c);
+int launch_fft_x_fwd_rhs(pll_context* c);
 int launch_fft_x_bwd(pll_context* c);
 int launch_fft_y_fwd(pll_context* c);
 int launch_fft_y_bwd(pll_context* c);
