@@ -42,11 +42,22 @@ DEFAULT_WORKLOAD = "retau550_768x384x384"
 # bulk mean of u rides along); the separate bulk mean reads u; the force update
 # reads and writes u.
 BYTES_PER_POINT_FP64 = {
-    "rk1": 48, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 80, "fft_x_fwd": 16, "fft_y_fwd": 16,
-    "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "correct_vel": 56, "mean_partial": 8,
-    "force_add": 16,
+    "rk1": 48, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 80, "fft_x_fwd": 16, "fft_x_fwd_rhs": 32,
+    "fft_y_fwd": 16, "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "correct_vel": 56,
+    "mean_partial": 8, "force_add": 16,
 }
 STEP_BYTES_FP64 = 312
+
+# HBM bytes per point per launch measured with rocprofv3 on the 768x384x384
+# workload (2 x FETCH_SIZE + WRITE_SIZE, the scale of a 1 GiB device copy);
+# profiles/retau550_768x384x384_step_kernels.md holds the runs.  Reported as
+# the roofline entry's `traffic` for that workload only.
+MEASURED_HBM_B_PER_POINT = {
+    "fp64": {"rk1": 51.2, "rk2_rhs": 87.9, "fft_x_fwd": 16.1, "fft_y_fwd": 16.0, "solve_trid": 38.7,
+             "fft_y_bwd": 16.0, "fft_x_bwd": 16.0, "correct_vel": 60.9},
+    "fp32_mixed": {"rk1": 25.1, "rk2": 36.9, "fft_x_fwd_rhs": 20.9, "fft_y_fwd": 8.0,
+                   "solve_trid": 22.6, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
+}
 
 
 def parse():
@@ -157,13 +168,16 @@ def measure(wl, precision, steps, warmup):
     dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: times[k][0])
     d_avg_s = times[dom][0] / times[dom][1] * 1e-3
     achieved = BYTES_PER_POINT_FP64[dom] * scale * npts / d_avg_s / 1e9
+    measured = MEASURED_HBM_B_PER_POINT.get(precision, {}).get(dom)
+    traffic = (round(measured * npts) if measured is not None and wl == DEFAULT_WORKLOAD
+               else None)
     step_gbs = STEP_BYTES_FP64 * scale * npts / (ms_step * 1e-3) / 1e9
     return {
         "value": round(npts * steps / sec / 1e6, 2),
         "ms_per_step": round(ms_step, 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "step_achieved": round(step_gbs, 1),
+                     "traffic": traffic, "step_achieved": round(step_gbs, 1),
                      "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
                      "bytes_per_point_step": STEP_BYTES_FP64 * scale},
         "kernels": kernels,

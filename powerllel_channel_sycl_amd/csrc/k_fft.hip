@@ -469,7 +469,10 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xf(Geom g, cons
 }
 
 // execute_fft_bwd_xpen (src/mod_fft.cc:90-120): perm rows -> p interior * norm.
-template <typename T, int L>
+// With HALO (the step sequence, where updateBoundP follows at once) a row is
+// stored as aligned pairs p(0..nx+1), the two x-halo cells holding their
+// periodic values p(nx) and p(1) -- what updateBoundP writes there.
+template <typename T, int L, bool HALO>
 __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xb(Geom g, const T* __restrict__ spec,
                                                                T* __restrict__ p,
                                                                const cpx<T>* __restrict__ twL,
@@ -483,12 +486,13 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xb(Geom g, cons
   if (threadIdx.x < NB) {
     const long r = row0 + threadIdx.x;
     const int j = (int)(r % g.ny), k = (int)(r / g.ny);
-    rowbase[threadIdx.x] = r < nrows ? g.idx(1, j + 1, k + 1) : -1;
+    rowbase[threadIdx.x] = r < nrows ? g.idx(HALO ? 0 : 1, j + 1, k + 1) : -1;
   }
-  for (int e = threadIdx.x; e < NB * N; e += NT) {
-    const int s = e / N, i = e % N;
+  const cpx<T>* specc = reinterpret_cast<const cpx<T>*>(spec);
+  for (int e = threadIdx.x; e < NB * L; e += NT) {
+    const int s = e / L, kk = e % L;
     const long r = row0 + s;
-    real_at(buf, s * N + i) = r < nrows ? spec[r * N + i] : T(0);
+    buf[pad(s * L + kk)] = r < nrows ? specc[r * L + kk] : cpx<T>{T(0), T(0)};
   }
   __syncthreads();
   const int s = threadIdx.x / TPS, jj = threadIdx.x % TPS;
@@ -517,10 +521,23 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xb(Geom g, cons
   for (int u = 0; u < RMIN; ++u) buf[pad(s * L + jj + u * TPS)] = z[u];
   __syncthreads();
   stockham<T, +1, L, 1>(buf, twL, s * L, jj);
-  for (int e = threadIdx.x; e < NB * N; e += NT) {
-    const int s2 = e / N, i = e % N;
-    const long b = rowbase[s2];
-    if (b >= 0) p[b + i] = norm * real_at(buf, s2 * N + i);
+  if (HALO) {
+    // pair q = (p(2q), p(2q+1)); p(i) is element i-1 of the row, p(0) = p(nx), p(nx+1) = p(1)
+    constexpr int NP = L + 1;
+    for (int e = threadIdx.x; e < NB * NP; e += NT) {
+      const int s2 = e / NP, q = e % NP;
+      const long b = rowbase[s2];
+      if (b < 0) continue;
+      const int e0 = q == 0 ? N - 1 : 2 * q - 1, e1 = q == L ? 0 : 2 * q;
+      cpx<T> v{norm * real_at(buf, s2 * N + e0), norm * real_at(buf, s2 * N + e1)};
+      reinterpret_cast<cpx<T>*>(p + b)[q] = v;
+    }
+  } else {
+    for (int e = threadIdx.x; e < NB * N; e += NT) {
+      const int s2 = e / N, i = e % N;
+      const long b = rowbase[s2];
+      if (b >= 0) p[b + i] = norm * real_at(buf, s2 * N + i);
+    }
   }
 }
 
@@ -534,9 +551,10 @@ __global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yf(Geom g, T* _
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = (int)(b % (unsigned)ntiles), k = (int)(b / (unsigned)ntiles);
   T* plane = spec + (long)k * g.nx * g.ny + (long)tile * W;
-  for (int e = threadIdx.x; e < L * W; e += NT) {
-    const int j = e / W, c = e % W;
-    real_at(buf, j * W + c) = plane[(long)j * g.nx + c];
+  // a slot holds the values of columns (2q, 2q+1) of one row: one 2-element load
+  for (int e = threadIdx.x; e < L * Q; e += NT) {
+    const int j = e / Q, q2 = e % Q;
+    buf[pad(j * Q + q2)] = reinterpret_cast<const cpx<T>*>(plane + (long)j * g.nx)[q2];
   }
   __syncthreads();
   const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
@@ -568,9 +586,10 @@ __global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yb(Geom g, T* _
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = (int)(b % (unsigned)ntiles), k = (int)(b / (unsigned)ntiles);
   T* plane = spec + (long)k * g.nx * g.ny + (long)tile * W;
-  for (int e = threadIdx.x; e < L * W; e += NT) {
-    const int j = e / W, c = e % W;
-    real_at(buf, j * W + c) = plane[(long)j * g.nx + c];
+  // a slot holds the values of columns (2q, 2q+1) of one row: one 2-element load
+  for (int e = threadIdx.x; e < L * Q; e += NT) {
+    const int j = e / Q, q2 = e % Q;
+    buf[pad(j * Q + q2)] = reinterpret_cast<const cpx<T>*>(plane + (long)j * g.nx)[q2];
   }
   __syncthreads();
   const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
@@ -596,9 +615,9 @@ __global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yb(Geom g, T* _
   for (int u = 0; u < RMIN; ++u) buf[pad((jj + u * TPS) * Q + q)] = z[u];
   __syncthreads();
   stockham<T, +1, L, Q>(buf, twL, q, jj);
-  for (int e = threadIdx.x; e < L * W; e += NT) {
-    const int j = e / W, c = e % W;
-    plane[(long)j * g.nx + c] = real_at(buf, j * W + c);
+  for (int e = threadIdx.x; e < L * Q; e += NT) {
+    const int j = e / Q, q2 = e % Q;
+    reinterpret_cast<cpx<T>*>(plane + (long)j * g.nx)[q2] = buf[pad(j * Q + q2)];
   }
 }
 
@@ -745,9 +764,14 @@ int x_fixed(pll_context* c, bool fwd, bool rhs = false) {
   } else {
     const double norm = 1.0 / (c->g.nx * c->g.ny);  // src/mod_fft.cc:57
     TimerScope ts(c, K_FFT_X_BWD);
-    hipLaunchKernelGGL((k_xb<T, L>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
-                       static_cast<const T*>(c->spec), static_cast<T*>(c->fld[PLL_P]), tw, rtw,
-                       nrows, (T)norm);
+    if (rhs)  // inside the step: updateBoundP follows, x halos may be written
+      hipLaunchKernelGGL((k_xb<T, L, true>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+                         static_cast<const T*>(c->spec), static_cast<T*>(c->fld[PLL_P]), tw, rtw,
+                         nrows, (T)norm);
+    else
+      hipLaunchKernelGGL((k_xb<T, L, false>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
+                         static_cast<const T*>(c->spec), static_cast<T*>(c->fld[PLL_P]), tw, rtw,
+                         nrows, (T)norm);
   }
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
@@ -867,6 +891,22 @@ int x_rhs_t(pll_context* c) {
 }
 
 bool fft_x_rhs_available(const pll_context* c) { return c->x_fixed; }
+
+template <typename T>
+int x_bwd_halo_t(pll_context* c) {
+  switch (c->g.nx / 2) {
+#define PLL_CASE(n) case n: return x_fixed<T, n>(c, false, true);
+    PLL_X_SIZES(PLL_CASE)
+#undef PLL_CASE
+    default: break;
+  }
+  return x_bwd_generic<T>(c);
+}
+
+int launch_fft_x_bwd_step(pll_context* c) {
+  if (!c->x_fixed) return launch_fft_x_bwd(c);
+  return c->prec == PLL_FP64 ? x_bwd_halo_t<double>(c) : x_bwd_halo_t<float>(c);
+}
 
 int launch_fft_x_fwd_rhs(pll_context* c) {
   if (!c->x_fixed) return fail(PLL_ERR_UNSUPPORTED, "no fixed-length x transform for calcRHS fusion");

@@ -131,6 +131,8 @@ template <bool CRF, class A>
 __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W, const RkCoef& q,
                                               const RkUniform& cu, float& iu, float& iv,
                                               float& iw) {
+  // the FP32 form is not held to the reference's rounding: let products fuse into FMAs
+#pragma clang fp contract(fast)
   {
     const float q0 = U(0, 0, 0), qe = U(1, 0, 0), qw = U(-1, 0, 0), qn = U(0, 1, 0);
     const float qs = U(0, -1, 0), qt = U(0, 0, 1), qb = U(0, 0, -1);
@@ -199,6 +201,14 @@ constexpr int kRkTy = 8;   // tile rows along y
 // (src/mod_updateBound.cc:11-144) -- so the result is bit-identical to forcing,
 // updating the boundary and then running the stage.  Stage 2 adds the
 // deferred forcing to the point value of u it reads.
+//
+// With `corr` the previous step's projection is deferred as well: the stored
+// velocity is the predictor u* and every value read becomes
+// u* - dt*grad(p) (src/mod_calcVel.cc:272-274) before the force and the
+// boundary rules apply; p carries valid halos (updateBoundP ran).  With `acc`
+// stage 2 also sums its new u times dzflzi(k) in FP64 into partial[block]: the
+// bulk mean of the next projected velocity, since the projection changes every
+// x-row's sum only by p(nx+1) - p(1) = 0.
 template <typename T, int STAGE>
 __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict__ su,
                                                      const T* __restrict__ sv,
@@ -210,7 +220,11 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
                                                      const double* __restrict__ force,
                                                      int forced_mask,
                                                      const RkCoef* __restrict__ coef,
-                                                     RkUniform cu, int crf) {
+                                                     RkUniform cu, int crf,
+                                                     const T* __restrict__ pp, int corr,
+                                                     T dtdxi, T dtdyi, int acc,
+                                                     const double* __restrict__ dzflzi,
+                                                     double* __restrict__ partial) {
   constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY, NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
   __shared__ T sh[3][3][PL];  // [field][ring slot][tile cell]
@@ -249,11 +263,21 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
     const bool ghost = fold && (kk == 0 || kk == g.nz + 1);
     const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
     const bool wzero = fold && (kk == 0 || kk >= g.nz);
+    // the stage-1 source is the stored velocity; stage 2's source u1 is not projected
+    const bool pj = STAGE == 1 && corr;
+    const T dzci = pj ? m.dzc_inv[ks] : T(0);
 #pragma unroll
     for (int n = 0; n < NPF; ++n)
       if (gaddr[n] >= 0) {
         const long a = gaddr[n] + g.sxy * ks;
         T qu = su[a], qv = sv[a], qw = wzero ? T(0) : sw[a];
+        if (pj) {
+          const T p0 = pp[a];
+          qu = qu - (pp[a + 1] - p0) * dtdxi;
+          qv = qv - (pp[a + g.sx] - p0) * dtdyi;
+          // same association as k_correct: ((p(k+1) - p(k)) * dt) * dzc_inv(k)
+          if (!wzero) qw = qw - (pp[a + g.sxy] - p0) * dt * dzci;
+        }
         if (fold) {
           if (fu) qu = qu + f_u;
           if (fv) qv = qv + f_v;
@@ -277,7 +301,11 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
         sh[2][slot][cell[n]] = pre[2][n];
       }
   };
-  if (k0 > k1) return;
+  if (k0 > k1) {
+    if (acc && threadIdx.x == 0) partial[b] = 0.0;
+    return;
+  }
+  double msum = 0.0;  // FP64 partial of the bulk mean (acc)
   fetch(k0 - 1);
   commit(0);
   fetch(k0);
@@ -300,6 +328,12 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
         ou = du[a];
         ov = dv[a];
         ow = dw[a];
+        if (corr) {
+          const T p0 = pp[a];
+          ou = ou - (pp[a + 1] - p0) * dtdxi;
+          ov = ov - (pp[a + g.sx] - p0) * dtdyi;
+          ow = ow - (pp[a + g.sxy] - p0) * dt * m.dzc_inv[k];
+        }
         if (defer) {
           if (fu) ou = ou + f_u;
           if (fv) ov = ov + f_v;
@@ -315,9 +349,11 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
           dv[a] = V(0, 0, 0) + dt * tv;
           dw[a] = W(0, 0, 0) + dt * tw;
         } else {
-          du[a] = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
+          const T nu = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
+          du[a] = nu;
           dv[a] = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
           dw[a] = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
+          if (acc) msum += (double)nu * dzflzi[k];
         }
       } else {
         const RkCoef q = coef[k];
@@ -331,9 +367,11 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
           dv[a] = V(0, 0, 0) + iv;
           dw[a] = W(0, 0, 0) + iw;
         } else {
-          du[a] = (U(0, 0, 0) + ou) * 0.5f + iu;
+          const T nu = (U(0, 0, 0) + ou) * 0.5f + iu;
+          du[a] = nu;
           dv[a] = (V(0, 0, 0) + ov) * 0.5f + iv;
           dw[a] = (W(0, 0, 0) + ow) * 0.5f + iw;
+          if (acc) msum += (double)nu * dzflzi[k];
         }
       }
     }
@@ -344,6 +382,18 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
     sm = s0;
     s0 = sp;
     sp = t;
+  }
+  if (acc) {
+    // the ring is free after the last barrier of the loop
+    double* red = reinterpret_cast<double*>(&sh[0][0][0]);
+    static_assert(sizeof(sh) >= NT * sizeof(double), "reduction scratch");
+    red[threadIdx.x] = msum;
+    __syncthreads();
+    for (int h = NT / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[b] = red[0];
   }
 }
 
@@ -1033,8 +1083,15 @@ int rk_coef_setup(pll_context* c) {
 }
 
 template <typename T>
-int rk_t(pll_context* c, int stage, int fold) {
+int rk_t(pll_context* c, int stage, int fold, int corr, int acc) {
   const TileGrid t = tile_grid(c);
+  if (acc && (long)t.blocks() > c->npartial)
+    return fail(PLL_ERR_STATE, "reduction scratch smaller than the stencil grid");
+  // the projected loads read p at the periodic image of halo cells, which only
+  // the folded (image-addressed) loads provide
+  if (corr && !fold) return fail(PLL_ERR_STATE, "deferred projection needs folded loads");
+  const double* dzflzi = c->d_mesh64 + 3 * (c->g.nz + 2);
+  const T dtdxi = (T)(c->dt * c->dx_inv), dtdyi = (T)(c->dt * c->dy_inv);  // src/mod_calcVel.cc:256-258
   const MeshT<T> m = mesh_of<T>(c);
   const T bc_u = (T)(0.0 - c->u_crf);  // src/mod_updateBound.cc:80
   const double* force = c->d_scalar + 3;
@@ -1047,16 +1104,26 @@ int rk_t(pll_context* c, int stage, int fold) {
                        F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
                        F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
-                       bc_u, force, forced_mask(c), coef, cu, crf);
+                       bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr, dtdxi,
+                       dtdyi, 0, dzflzi, c->d_partial);
   } else {
     TimerScope ts(c, K_RK2);
     hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
                        F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
                        F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, 0, fold,
-                       bc_u, force, forced_mask(c), coef, cu, crf);
+                       bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr, dtdxi,
+                       dtdyi, acc, dzflzi, c->d_partial);
   }
   PLL_HIP_TRY(hipGetLastError());
+  if (acc) {
+    // forceVel's bulk mean and force of u (src/mod_calcVel.cc:287,297)
+    TimerScope ts(c, K_MEAN_FINAL);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kRedThreads), 0, c->stream, c->d_partial,
+                       (long)t.blocks(), (double)c->g.nx, (double)c->g.ny, c->d_scalar + 0,
+                       c->vel_force[0], c->d_scalar + 3);
+    PLL_HIP_TRY(hipGetLastError());
+  }
   return PLL_OK;
 }
 
@@ -1295,8 +1362,9 @@ int nan_t(pll_context* c, int field, int* passed) {
 
 int rk_setup(pll_context* c) { return c->prec == PLL_FP64 ? PLL_OK : rk_coef_setup(c); }
 
-int launch_rk(pll_context* c, int stage, int fold) {
-  return PLL_DISPATCH(c, rk_t<double>(c, stage, fold), rk_t<float>(c, stage, fold));
+int launch_rk(pll_context* c, int stage, int fold, int corr, int acc) {
+  return PLL_DISPATCH(c, rk_t<double>(c, stage, fold, corr, acc),
+                      rk_t<float>(c, stage, fold, corr, acc));
 }
 int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
 int launch_rk2_rhs(pll_context* c, int defer) {

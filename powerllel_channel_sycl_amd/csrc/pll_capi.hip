@@ -303,21 +303,33 @@ int step_impl(pll_context* c, bool first, bool last) {
   const bool fused = !c->is_forced[1] && !c->is_forced[2];
   const int fold = (fused && !first) ? 1 : 0;
   int rc;
+  if (fused && c->prec != PLL_FP64 && fft_x_rhs_available(c)) {
+    // FP32-mixed: the projection of step n (:145), its forcing (:149) and
+    // boundary update (:153) are applied by step n+1's stage loads (corr);
+    // stage 2 sums the bulk mean of the predictor for that forcing, and
+    // calcRHS runs inside the forward x transform's loads (:133-137).
+    const int corr = fold;
+    const int acc = (!last && c->is_forced[0]) ? 1 : 0;
+    if ((rc = launch_rk(c, 1, fold, corr, 0))) return rc;    // :117
+    if ((rc = launch_bc_vel(c, 1))) return rc;               // :121
+    if ((rc = launch_rk(c, 2, fold, corr, acc))) return rc;  // :125
+    if ((rc = launch_bc_vel(c, 0))) return rc;               // :129
+    if ((rc = launch_fft_x_fwd_rhs(c))) return rc;
+    if ((rc = launch_fft_y_fwd(c))) return rc;
+    if ((rc = launch_trid(c))) return rc;
+    if ((rc = launch_fft_y_bwd(c))) return rc;
+    if ((rc = launch_fft_x_bwd_step(c))) return rc;
+    if ((rc = launch_bc_p(c))) return rc;                    // :141
+    if (!last) return PLL_OK;
+    if ((rc = launch_correct_vel(c, c->is_forced[0] != 0))) return rc;  // :145 (+ mean of :149)
+    if (c->is_forced[0] && (rc = launch_add_force_u(c))) return rc;     // :149
+    return launch_bc_vel(c, 0);                                          // :153
+  }
   if ((rc = launch_rk(c, 1, fold))) return rc;    // src/main.cc:117
   if (!fused || c->prec != PLL_FP64) {
-    // FP32-mixed: separate kernels are faster (the fused one is latency-bound there)
     if ((rc = launch_bc_vel(c, 1))) return rc;    // :121
     if ((rc = launch_rk(c, 2, fold))) return rc;  // :125
     if ((rc = launch_bc_vel(c, 0))) return rc;    // :129
-    if (c->prec != PLL_FP64 && fft_x_rhs_available(c)) {
-      // :133-137 with calcRHS computed inside the forward x transform's loads
-      if ((rc = launch_fft_x_fwd_rhs(c))) return rc;
-      if ((rc = launch_fft_y_fwd(c))) return rc;
-      if ((rc = launch_trid(c))) return rc;
-      if ((rc = launch_fft_y_bwd(c))) return rc;
-      if ((rc = launch_fft_x_bwd(c))) return rc;
-      return finish_step(c, fused, last);
-    }
     if ((rc = launch_calc_rhs(c))) return rc;     // :133
   } else {
     // :121-133 in one kernel: the u1 boundary is rebuilt on load, the middle
@@ -326,7 +338,11 @@ int step_impl(pll_context* c, bool first, bool last) {
     if ((rc = launch_rk2_rhs(c, fold))) return rc;
     if (last && (rc = launch_bc_vel(c, 1))) return rc;
   }
-  if ((rc = poisson_impl(c))) return rc;          // :137
+  if ((rc = launch_fft_x_fwd(c))) return rc;      // :137
+  if ((rc = launch_fft_y_fwd(c))) return rc;
+  if ((rc = launch_trid(c))) return rc;
+  if ((rc = launch_fft_y_bwd(c))) return rc;
+  if ((rc = launch_fft_x_bwd_step(c))) return rc;
   return finish_step(c, fused, last);
 }
 

@@ -169,7 +169,9 @@ struct TimerScope {
 // stencils and pointwise kernels (k_stencil.hip)
 // fold: stage 1 applies the previous step's deferred force and rebuilds the
 // halos on load; stage 2 adds the deferred force to its point value of u.
-int launch_rk(pll_context* c, int stage, int fold);
+// corr: apply the deferred projection on load; acc (stage 2): bulk mean of the
+// new u into d_scalar[0], force into d_scalar[3]
+int launch_rk(pll_context* c, int stage, int fold, int corr = 0, int acc = 0);
 int rk_setup(pll_context* c);  // per-plane coefficients of the FP32-mixed stencil
 int launch_calc_rhs(pll_context* c);
 // RK stage 2 fused with calcRHS (step sequence only): writes the new u, v, w to
@@ -197,6 +199,8 @@ int launch_fft_x_fwd(pll_context* c);
 // calcRHS fused into the forward x transform (reads u, v, w; p is not written)
 bool fft_x_rhs_available(const pll_context* c);
 int launch_fft_x_fwd_rhs(pll_context* c);
+// inverse x transform inside the step (x halos of p written with their periodic values)
+int launch_fft_x_bwd_step(pll_context* c);
 int launch_fft_x_bwd(pll_context* c);
 int launch_fft_y_fwd(pll_context* c);
 int launch_fft_y_bwd(pll_context* c);

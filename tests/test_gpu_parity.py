@@ -245,7 +245,10 @@ def test_full_step_parity(name, steps):
 @pytest.mark.parametrize("name,precision", [("r3", "fp64"), ("c1", "fp64"), ("r3", "fp32_mixed")])
 def test_multistep_folding_is_bitexact(name, precision):
     """pll_step(n) folds each step's forcing and final boundary update into the
-    next step's loads; it must give the same bits as n single steps."""
+    next step's loads; in FP64 it must give the same bits as n single steps.
+    The FP32-mixed step also defers the projection into the next step's loads
+    and takes the forcing's bulk mean from the predictor (the projection leaves
+    every x-row sum unchanged), which moves the result by FP32 round-off only."""
     _print_case("folding", name, precision)
     o = make_oracle(name)
     u, v, w, p = (o.field() for _ in range(4))
@@ -264,7 +267,11 @@ def test_multistep_folding_is_bitexact(name, precision):
                     ch.step(1)
             outs.append([ch.download(f) for f in VEL + ("p",)])
     for a, b, fld in zip(outs[0], outs[1], VEL + ("p",)):
-        np.testing.assert_array_equal(a, b, err_msg=fld)
+        if precision == "fp64":
+            np.testing.assert_array_equal(a, b, err_msg=fld)
+        else:
+            # FP32 fields of magnitude ~1: a few units in the last place
+            np.testing.assert_allclose(a, b, rtol=0, atol=2e-6, err_msg=fld)
 
 
 def test_entry_point_sequence_matches_step():
