@@ -350,12 +350,6 @@ constexpr int y_seqs() {
   return q;
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of logical blocks so neighbouring tiles share its L2.
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
-  const unsigned q = n / 8, r = n % 8, x = b % 8, l = b / 8;
-  return x * q + (x < r ? x : r) + l;
-}
 
 template <typename T, int S, int L, int R, int NS, int ES>
 __device__ __forceinline__ void stockham_pass(cpx<T>* buf, const cpx<T>* __restrict__ twL,

@@ -176,12 +176,6 @@ __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W
   }
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of logical blocks so neighbouring tiles share its L2.
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
-  const unsigned q = n / 8, r = n % 8, x = b % 8, l = b / 8;
-  return x * q + (x < r ? x : r) + l;
-}
 
 constexpr int kRkTx = 64;  // tile width along x (one wavefront per row)
 constexpr int kRkTy = 8;   // tile rows along y
@@ -227,6 +221,11 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
                                                      double* __restrict__ partial) {
   constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY, NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
+  // the deferred projection and the predictor's mean serve the FP32-mixed step
+  // only; compiling them out of FP64 keeps its register count and occupancy
+  constexpr bool kF32 = std::is_same<T, float>::value;
+  corr = kF32 ? corr : 0;
+  acc = kF32 ? acc : 0;
   __shared__ T sh[3][3][PL];  // [field][ring slot][tile cell]
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
@@ -259,6 +258,8 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
     gaddr[n] = inside ? gx + g.sx * gy : -1;
   }
   T pre[3][NPF];
+  T pabove[NPF];      // p of plane ks+1 from the previous fetch (corr)
+  int pabove_k = -1;  // the plane pabove belongs to
   auto fetch = [&](int kk) {
     const bool ghost = fold && (kk == 0 || kk == g.nz + 1);
     const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
@@ -266,17 +267,19 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
     // the stage-1 source is the stored velocity; stage 2's source u1 is not projected
     const bool pj = STAGE == 1 && corr;
     const T dzci = pj ? m.dzc_inv[ks] : T(0);
+    const bool reuse = pabove_k == ks;
 #pragma unroll
     for (int n = 0; n < NPF; ++n)
       if (gaddr[n] >= 0) {
         const long a = gaddr[n] + g.sxy * ks;
         T qu = su[a], qv = sv[a], qw = wzero ? T(0) : sw[a];
         if (pj) {
-          const T p0 = pp[a];
+          const T p0 = reuse ? pabove[n] : pp[a];
+          pabove[n] = pp[a + g.sxy];
           qu = qu - (pp[a + 1] - p0) * dtdxi;
           qv = qv - (pp[a + g.sx] - p0) * dtdyi;
           // same association as k_correct: ((p(k+1) - p(k)) * dt) * dzc_inv(k)
-          if (!wzero) qw = qw - (pp[a + g.sxy] - p0) * dt * dzci;
+          if (!wzero) qw = qw - (pabove[n] - p0) * dt * dzci;
         }
         if (fold) {
           if (fu) qu = qu + f_u;
@@ -291,6 +294,7 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
         pre[1][n] = qv;
         pre[2][n] = qw;
       }
+    pabove_k = pj ? ks + 1 : -1;
   };
   auto commit = [&](int slot) {
 #pragma unroll
@@ -315,9 +319,28 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
   __syncthreads();
   int sm = 0, s0 = 1, sp = 2;
   const int c = (ty + 1) * SX + tx + 1;
+  // stage 2's point values and, with corr, the pressures around the point
+  // (fetching them a plane ahead costs more in occupancy than it saves)
+  struct Point {
+    T u, v, w, p0, pe, pn, pt;
+  };
+  auto load_point = [&](int kk, Point& q) {
+    const long a = g.idx(i, j, kk);
+    q.u = du[a];
+    q.v = dv[a];
+    q.w = dw[a];
+    if (corr) {
+      q.p0 = pp[a];
+      q.pe = pp[a + 1];
+      q.pn = pp[a + g.sx];
+      q.pt = pp[a + g.sxy];
+    }
+  };
+  Point qc{};
   for (int k = k0; k <= k1; ++k) {
     const bool more = k + 2 <= k1 + 1;
     if (more) fetch(k + 2);
+    if (STAGE == 2 && active) load_point(k, qc);
     if (active) {
       const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
       const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
@@ -325,14 +348,13 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
       const long a = g.idx(i, j, k);
       T ou = T(0), ov = T(0), ow = T(0);
       if (STAGE == 2) {
-        ou = du[a];
-        ov = dv[a];
-        ow = dw[a];
+        ou = qc.u;
+        ov = qc.v;
+        ow = qc.w;
         if (corr) {
-          const T p0 = pp[a];
-          ou = ou - (pp[a + 1] - p0) * dtdxi;
-          ov = ov - (pp[a + g.sx] - p0) * dtdyi;
-          ow = ow - (pp[a + g.sxy] - p0) * dt * m.dzc_inv[k];
+          ou = ou - (qc.pe - qc.p0) * dtdxi;
+          ov = ov - (qc.pn - qc.p0) * dtdyi;
+          ow = ow - (qc.pt - qc.p0) * dt * m.dzc_inv[k];
         }
         if (defer) {
           if (fu) ou = ou + f_u;

@@ -9,10 +9,15 @@
 // distinct pair: (nx/2+1)*(ny/2+1)*nz values, built on the host with the
 // reference's arithmetic, so every pivot is bit-identical to the reference's.
 //
-// One thread per (i,j) column, coalesced over i, sequential in k.  The
-// arithmetic is FP64 in both precision modes (the FP32-mixed path stores the
-// work array in FP32 but eliminates in FP64, which covers the kx=ky=0 and the
-// near-singular low-wavenumber columns).
+// One thread per (i,j) column, coalesced over i, sequential in k; many
+// columns per CU hide the latency of the dependent sweeps.  (Staging whole
+// columns in LDS saves the second pass over the work array but leaves a CU
+// only ~50 FP64 columns of 384 planes: the sweeps then run latency-bound,
+// 3x slower at 768x384x384.)  The arithmetic is FP64 in both precision modes
+// (the FP32-mixed path stores the work array in FP32 but eliminates in FP64,
+// which covers the kx=ky=0 and the near-singular low-wavenumber columns).
+// FP64 keeps the reference's divisions (bit-exact); FP32-mixed stores
+// reciprocal pivots and multiplies (RECIP).
 #include "pll_internal.hpp"
 
 #include <cmath>
@@ -31,7 +36,7 @@ __host__ __device__ __forceinline__ int compact_index(int i) { return i <= 1 ? i
 // each sweep waits on HBM latency once per D planes rather than once per plane.
 constexpr int kD = 8;
 
-template <typename T>
+template <typename T, bool RECIP>
 __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __restrict__ var,
                                                    const double* __restrict__ a,
                                                    const double* __restrict__ c,
@@ -68,7 +73,7 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
     for (int d = 0; d < kD; ++d) {
       const int k = kb + d;
       if (k < nz) {
-        const double a_tmp = a[k] / xb[buf][d];
+        const double a_tmp = RECIP ? a[k] * xb[buf][d] : a[k] / xb[buf][d];
         const double cur = (double)xv[buf][d] - a_tmp * prev;
         V(k) = (T)cur;
         prev = cur;
@@ -84,7 +89,10 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
   }
   // last row, src/mod_poisson_solver.cc:341-346
   const double b_val = B(nz - 1);
-  prev = (b_val != 0) ? prev / b_val : 0.0;
+  if (RECIP)
+    prev = prev * b_val;  // the reciprocal table holds 0 for a zero pivot
+  else
+    prev = (b_val != 0) ? prev / b_val : 0.0;
   V(nz - 1) = (T)prev;
   // back substitution, src/mod_poisson_solver.cc:348-354:
   // var[k] = (var[k] - c[k] * var[k+1]) / b[k], k = nz-2..0
@@ -103,7 +111,8 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
     for (int d = 0; d < kD; ++d) {
       const int k = kt - d;
       if (k >= 0) {
-        const double cur = ((double)xv[buf][d] - c[k] * prev) / xb[buf][d];
+        const double r = (double)xv[buf][d] - c[k] * prev;
+        const double cur = RECIP ? r * xb[buf][d] : r / xb[buf][d];
         V(k) = (T)cur;
         prev = cur;
       }
@@ -117,6 +126,7 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
     if (kt - kD >= 0) run_bwd(1, kt - kD);
   }
 }
+
 
 }  // namespace
 
@@ -162,6 +172,10 @@ int trid_setup(pll_context* c) {
       }
     }
   if (c->cfg.pin_singular) piv[(size_t)(nz - 1) * P] = 0.0;
+  // FP32-mixed multiplies by reciprocal pivots (0 stands for a zero pivot:
+  // the reference's b == 0 branch gives 0, src/mod_poisson_solver.cc:341-346)
+  if (c->prec != PLL_FP64)
+    for (double& x : piv) x = x != 0.0 ? 1.0 / x : 0.0;
 
   PLL_HIP_TRY(hipMalloc(&c->d_a, nz * sizeof(double)));
   PLL_HIP_TRY(hipMalloc(&c->d_c, nz * sizeof(double)));
@@ -173,16 +187,16 @@ int trid_setup(pll_context* c) {
 }
 
 int launch_trid(pll_context* c) {
-  const dim3 grid((c->g.nx + kTx - 1) / kTx, (c->g.ny + kTy - 1) / kTy);
   TimerScope ts(c, K_TRID);
+  const dim3 grid((c->g.nx + kTx - 1) / kTx, (c->g.ny + kTy - 1) / kTy);
   if (c->prec == PLL_FP64)
-    hipLaunchKernelGGL((k_trid<double>), grid, dim3(kTx, kTy), 0, c->stream, c->g.nx, c->g.ny,
-                       c->g.nz, static_cast<double*>(c->spec), c->d_a, c->d_c, c->d_piv, c->cxn,
-                       c->cyn);
+    hipLaunchKernelGGL((k_trid<double, false>), grid, dim3(kTx, kTy), 0, c->stream, c->g.nx,
+                       c->g.ny, c->g.nz, static_cast<double*>(c->spec), c->d_a, c->d_c, c->d_piv,
+                       c->cxn, c->cyn);
   else
-    hipLaunchKernelGGL((k_trid<float>), grid, dim3(kTx, kTy), 0, c->stream, c->g.nx, c->g.ny,
-                       c->g.nz, static_cast<float*>(c->spec), c->d_a, c->d_c, c->d_piv, c->cxn,
-                       c->cyn);
+    hipLaunchKernelGGL((k_trid<float, true>), grid, dim3(kTx, kTy), 0, c->stream, c->g.nx,
+                       c->g.ny, c->g.nz, static_cast<float*>(c->spec), c->d_a, c->d_c, c->d_piv,
+                       c->cxn, c->cyn);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }

@@ -146,7 +146,7 @@ void release(pll_context* c) {
   for (double*& p : c->d_stat)
     if (p) (void)hipFree(p);
   void* bufs[] = {c->spec,   c->d_mesh, c->d_mesh64, c->d_a,    c->d_c,       c->d_piv,
-                  c->d_piv32, c->d_ac32, c->d_lowk,  c->d_tw_x, c->d_tw_y,    c->d_rtw_x, c->d_twl_x, c->d_twl_y, c->d_rkcoef,
+                  c->d_tw_x, c->d_tw_y,    c->d_rtw_x, c->d_twl_x, c->d_twl_y, c->d_rkcoef,
                   c->d_partial, c->d_scalar, c->d_red};
   for (void* p : bufs)
     if (p) (void)hipFree(p);

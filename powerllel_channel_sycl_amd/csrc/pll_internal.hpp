@@ -37,6 +37,13 @@ struct FftPlan {
   int twoff[kMaxPass];  // offset of the pass' twiddles: tw[off + k*(R-1) + r-1]
 };
 
+// Blocks are dispatched round-robin over the 8 XCDs of the GPU; this gives each
+// XCD a contiguous range of logical blocks, so neighbouring tiles share its L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
+  const unsigned q = n / 8, r = n % 8, x = b % 8, l = b / 8;
+  return x * q + (x < r ? x : r) + l;
+}
+
 // Kernel identifiers for the per-kernel timers.
 enum KernelId {
   K_RK1 = 0,
@@ -97,11 +104,7 @@ struct pll_context {
   int cxn = 0, cyn = 0;  // nx/2+1, ny/2+1
   double* d_a = nullptr;
   double* d_c = nullptr;
-  double* d_piv = nullptr;   // FP64 pivots, cxn*cyn*nz
-  float* d_piv32 = nullptr;  // FP32 pivots (FP32-mixed only)
-  float* d_ac32 = nullptr;   // FP32 a, c (FP32-mixed only)
-  double* d_lowk = nullptr;  // FP64 flag table per (cx,cy): 1 = solve in FP64 (FP32-mixed)
-  int nlowk = 0;
+  double* d_piv = nullptr;  // FP64 pivots (FP32-mixed: their reciprocals), cxn*cyn*nz
 
   // FFT plans and twiddles (in field precision)
   pll::FftPlan plan_x;  // complex length nx/2
