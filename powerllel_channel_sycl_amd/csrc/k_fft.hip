@@ -404,6 +404,10 @@ struct RhsSrc {
   const T* w;
   const T* dzf_inv;
   T dti, dtidxi, dtidyi;
+  // the velocity's boundary update did not run: the backward neighbours in
+  // the halo are read from their periodic images and w is the wall value 0
+  // on planes 0 and nz (src/mod_updateBound.cc:32-144)
+  int bcfold;
 };
 
 // execute_fft_fwd_xpen (src/mod_fft.cc:60-88), L = nx/2 points per row.  With
@@ -418,12 +422,19 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xf(Geom g, cons
   __shared__ cpx<T> buf[padded(NB * L)];
   __shared__ long rowbase[NB];
   __shared__ T rowdz[NB];
+  __shared__ long rowvs[NB];     // offset of the v row below (j-1, or its image)
+  __shared__ unsigned char roww[NB];  // bit 0: w below is the wall 0; bit 1: w is the wall 0
   const long row0 = (long)xcd_remap(blockIdx.x, gridDim.x) * NB;
   if (threadIdx.x < NB) {
     const long r = row0 + threadIdx.x;
     const int j = (int)(r % g.ny), k = (int)(r / g.ny);
     rowbase[threadIdx.x] = r < nrows ? g.idx(1, j + 1, k + 1) : -1;
-    if (RHS) rowdz[threadIdx.x] = r < nrows ? rs.dti * rs.dzf_inv[k + 1] : T(0);
+    if (RHS) {
+      rowdz[threadIdx.x] = r < nrows ? rs.dti * rs.dzf_inv[k + 1] : T(0);
+      const bool f = rs.bcfold != 0;
+      rowvs[threadIdx.x] = (f && j == 0) ? (long)(g.ny - 1) * g.sx : -g.sx;
+      roww[threadIdx.x] = (unsigned char)((f && k == 0 ? 1 : 0) | (f && k + 1 == g.nz ? 2 : 0));
+    }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < NB * N; e += NT) {
@@ -434,8 +445,12 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xf(Geom g, cons
       const long c = b + i;
       if (RHS) {
         // src/mod_calcRHS.cc:21-28, same association as k_calc_rhs
-        val = (rs.u[c] - rs.u[c - 1]) * rs.dtidxi + (rs.v[c] - rs.v[c - g.sx]) * rs.dtidyi +
-              (rs.w[c] - rs.w[c - g.sxy]) * rowdz[s];
+        const long cw = (i == 0 && rs.bcfold) ? b + (N - 1) : c - 1;  // u(0) = u(nx)
+        const unsigned char wf = roww[s];
+        const T wt = (wf & 2) ? T(0) : rs.w[c];
+        const T wb = (wf & 1) ? T(0) : rs.w[c - g.sxy];
+        val = (rs.u[c] - rs.u[cw]) * rs.dtidxi + (rs.v[c] - rs.v[c + rowvs[s]]) * rs.dtidyi +
+              (wt - wb) * rowdz[s];
       } else {
         val = p[c];
       }
@@ -746,7 +761,7 @@ int x_fixed(pll_context* c, bool fwd, bool rhs = false) {
     RhsSrc<T> rs{static_cast<const T*>(c->fld[PLL_U]), static_cast<const T*>(c->fld[PLL_V]),
                  static_cast<const T*>(c->fld[PLL_W]),
                  static_cast<const T*>(c->d_mesh) + (c->g.nz + 2),  // dzf_inv
-                 (T)dti, (T)(dti * c->dx_inv), (T)(dti * c->dy_inv)};
+                 (T)dti, (T)(dti * c->dx_inv), (T)(dti * c->dy_inv), c->rhs_bcfold};
     TimerScope ts(c, K_FFT_X_FWD_RHS);
     hipLaunchKernelGGL((k_xf<T, L, true>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
                        static_cast<const T*>(nullptr), static_cast<T*>(c->spec), tw, rtw, nrows, rs);

@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
                                                      const T* __restrict__ pp, int corr,
                                                      T dtdxi, T dtdyi, int acc,
                                                      const double* __restrict__ dzflzi,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial, int bcload) {
   constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY, NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
   // the deferred projection and the predictor's mean serve the FP32-mixed step
@@ -226,6 +226,9 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
   constexpr bool kF32 = std::is_same<T, float>::value;
   corr = kF32 ? corr : 0;
   acc = kF32 ? acc : 0;
+  // halo cells from the boundary rules on load: with the folded forcing, or
+  // alone (bcload) for a source whose boundary update was not run
+  bcload = (fold || (kF32 && bcload)) ? 1 : 0;
   __shared__ T sh[3][3][PL];  // [field][ring slot][tile cell]
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
     int gx = i0 + e % SX, gy = j0 + e / SX;
     cell[n] = e;
     const bool inside = e < PL && gx <= g.nx + 1 && gy <= g.ny + 1;
-    if (fold) {  // halo cells read their periodic image
+    if (bcload) {  // halo cells read their periodic image
       gx = gx == 0 ? g.nx : (gx == g.nx + 1 ? 1 : gx);
       gy = gy == 0 ? g.ny : (gy == g.ny + 1 ? 1 : gy);
     }
@@ -261,9 +264,9 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
   T pabove[NPF];      // p of plane ks+1 from the previous fetch (corr)
   int pabove_k = -1;  // the plane pabove belongs to
   auto fetch = [&](int kk) {
-    const bool ghost = fold && (kk == 0 || kk == g.nz + 1);
+    const bool ghost = bcload && (kk == 0 || kk == g.nz + 1);
     const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
-    const bool wzero = fold && (kk == 0 || kk >= g.nz);
+    const bool wzero = bcload && (kk == 0 || kk >= g.nz);
     // the stage-1 source is the stored velocity; stage 2's source u1 is not projected
     const bool pj = STAGE == 1 && corr;
     const T dzci = pj ? m.dzc_inv[ks] : T(0);
@@ -359,7 +362,8 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
         if (defer) {
           if (fu) ou = ou + f_u;
           if (fv) ov = ov + f_v;
-          if (fw) ow = (k == g.nz) ? T(0) : ow + f_w;
+          if (fw) ow = ow + f_w;
+          if (k == g.nz) ow = T(0);  // the wall value of the boundary update
         }
       }
       if constexpr (std::is_same<T, double>::value) {
@@ -1105,7 +1109,7 @@ int rk_coef_setup(pll_context* c) {
 }
 
 template <typename T>
-int rk_t(pll_context* c, int stage, int fold, int corr, int acc) {
+int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
   const TileGrid t = tile_grid(c);
   if (acc && (long)t.blocks() > c->npartial)
     return fail(PLL_ERR_STATE, "reduction scratch smaller than the stencil grid");
@@ -1127,7 +1131,7 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc) {
                        F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
                        bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr, dtdxi,
-                       dtdyi, 0, dzflzi, c->d_partial);
+                       dtdyi, 0, dzflzi, c->d_partial, fold);
   } else {
     TimerScope ts(c, K_RK2);
     hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
@@ -1135,7 +1139,7 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc) {
                        F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, 0, fold,
                        bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr, dtdxi,
-                       dtdyi, acc, dzflzi, c->d_partial);
+                       dtdyi, acc, dzflzi, c->d_partial, bcload);
   }
   PLL_HIP_TRY(hipGetLastError());
   if (acc) {
@@ -1384,9 +1388,9 @@ int nan_t(pll_context* c, int field, int* passed) {
 
 int rk_setup(pll_context* c) { return c->prec == PLL_FP64 ? PLL_OK : rk_coef_setup(c); }
 
-int launch_rk(pll_context* c, int stage, int fold, int corr, int acc) {
-  return PLL_DISPATCH(c, rk_t<double>(c, stage, fold, corr, acc),
-                      rk_t<float>(c, stage, fold, corr, acc));
+int launch_rk(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
+  return PLL_DISPATCH(c, rk_t<double>(c, stage, fold, corr, acc, bcload),
+                      rk_t<float>(c, stage, fold, corr, acc, bcload));
 }
 int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
 int launch_rk2_rhs(pll_context* c, int defer) {

@@ -308,19 +308,23 @@ int step_impl(pll_context* c, bool first, bool last) {
     // boundary update (:153) are applied by step n+1's stage loads (corr);
     // stage 2 sums the bulk mean of the predictor for that forcing, and
     // calcRHS runs inside the forward x transform's loads (:133-137).
+    // The boundary updates of u1 (:121) and u (:129) are not run either:
+    // stage 2 and calcRHS rebuild the halos they read from the boundary rules.
     const int corr = fold;
     const int acc = (!last && c->is_forced[0]) ? 1 : 0;
-    if ((rc = launch_rk(c, 1, fold, corr, 0))) return rc;    // :117
-    if ((rc = launch_bc_vel(c, 1))) return rc;               // :121
-    if ((rc = launch_rk(c, 2, fold, corr, acc))) return rc;  // :125
-    if ((rc = launch_bc_vel(c, 0))) return rc;               // :129
-    if ((rc = launch_fft_x_fwd_rhs(c))) return rc;
+    if ((rc = launch_rk(c, 1, fold, corr, 0))) return rc;       // :117
+    if ((rc = launch_rk(c, 2, fold, corr, acc, 1))) return rc;  // :125
+    c->rhs_bcfold = 1;
+    rc = launch_fft_x_fwd_rhs(c);
+    c->rhs_bcfold = 0;
+    if (rc) return rc;
     if ((rc = launch_fft_y_fwd(c))) return rc;
     if ((rc = launch_trid(c))) return rc;
     if ((rc = launch_fft_y_bwd(c))) return rc;
     if ((rc = launch_fft_x_bwd_step(c))) return rc;
     if ((rc = launch_bc_p(c))) return rc;                    // :141
     if (!last) return PLL_OK;
+    if ((rc = launch_bc_vel(c, 1))) return rc;                          // :121, u1's halos
     if ((rc = launch_correct_vel(c, c->is_forced[0] != 0))) return rc;  // :145 (+ mean of :149)
     if (c->is_forced[0] && (rc = launch_add_force_u(c))) return rc;     // :149
     return launch_bc_vel(c, 0);                                          // :153

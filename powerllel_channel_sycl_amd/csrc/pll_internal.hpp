@@ -115,6 +115,7 @@ struct pll_context {
   void* d_twl_x = nullptr;  // exp(-2 pi i t/L), L = nx/2 (fixed-length kernels)
   void* d_twl_y = nullptr;  // exp(-2 pi i t/L), L = ny
   bool x_fixed = false, y_fixed = false;
+  int rhs_bcfold = 0;  // calcRHS in the x transform reads halos through the boundary rules
   int x_batch = 1, y_tile = 2;
 
   // reductions
@@ -174,7 +175,9 @@ struct TimerScope {
 // halos on load; stage 2 adds the deferred force to its point value of u.
 // corr: apply the deferred projection on load; acc (stage 2): bulk mean of the
 // new u into d_scalar[0], force into d_scalar[3]
-int launch_rk(pll_context* c, int stage, int fold, int corr = 0, int acc = 0);
+// bcload (stage 2, FP32): rebuild the source's halos on load (u1's boundary
+// update not run)
+int launch_rk(pll_context* c, int stage, int fold, int corr = 0, int acc = 0, int bcload = 0);
 int rk_setup(pll_context* c);  // per-plane coefficients of the FP32-mixed stencil
 int launch_calc_rhs(pll_context* c);
 // RK stage 2 fused with calcRHS (step sequence only): writes the new u, v, w to
