@@ -34,18 +34,25 @@ WORKLOADS = {
 }
 DEFAULT_WORKLOAD = "retau550_768x384x384"
 
-# Algorithmic (compulsory) HBM bytes per grid point per launch, FP64; FP32 halves them.
-# RK1 reads u,v,w and writes u1,v1,w1; RK2 reads u1,v1,w1,u,v,w and writes u,v,w;
-# calcRHS reads u,v,w and writes p; RK2 fused with calcRHS (the FP64 step) reads
-# u1,v1,w1,u,v,w and writes u,v,w,p; each transform and the tridiagonal solve read
-# and write the work array once; correctVel reads p,u,v,w and writes u,v,w (the
-# bulk mean of u rides along); the separate bulk mean reads u; the force update
-# reads and writes u.
-BYTES_PER_POINT_FP64 = {
-    "rk1": 48, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 80, "fft_x_fwd": 16, "fft_x_fwd_rhs": 32,
-    "fft_y_fwd": 16, "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "correct_vel": 56,
-    "mean_partial": 8, "force_add": 16,
+# Algorithmic (compulsory) HBM bytes per grid point per launch.  FP64: RK1
+# reads u,v,w and writes u1,v1,w1; RK2 fused with calcRHS reads u1,v1,w1,u,v,w
+# and writes u,v,w,p; each transform and the tridiagonal solve read and write
+# the work array once; correctVel reads p,u,v,w and writes u,v,w (the bulk mean
+# of u rides along); the separate bulk mean reads u; the force update reads and
+# writes u.  FP32-mixed (4-byte fields): stage 1 also reads p (the previous
+# step's projection is applied on load), stage 2 reads u1,v1,w1,u,v,w,p and
+# writes u,v,w, and calcRHS runs inside the forward x transform (reads u,v,w,
+# writes the work array).
+BYTES_PER_POINT = {
+    "fp64": {"rk1": 48, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 80, "fft_x_fwd": 16,
+             "fft_y_fwd": 16, "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16,
+             "correct_vel": 56, "mean_partial": 8, "force_add": 16},
+    "fp32_mixed": {"rk1": 28, "rk2": 40, "calc_rhs": 16, "fft_x_fwd": 8, "fft_x_fwd_rhs": 16,
+                   "fft_y_fwd": 8, "solve_trid": 8, "fft_y_bwd": 8, "fft_x_bwd": 8,
+                   "correct_vel": 28, "mean_partial": 4, "force_add": 8},
 }
+# The algorithm's compulsory bytes per point per RK2 step (unfused, FP64): the
+# roofline of the whole step; FP32-mixed halves it.
 STEP_BYTES_FP64 = 312
 
 # HBM bytes per point per launch measured with rocprofv3 on the 768x384x384
@@ -53,10 +60,10 @@ STEP_BYTES_FP64 = 312
 # profiles/retau550_768x384x384_step_kernels.md holds the runs.  Reported as
 # the roofline entry's `traffic` for that workload only.
 MEASURED_HBM_B_PER_POINT = {
-    "fp64": {"rk1": 51.2, "rk2_rhs": 87.9, "fft_x_fwd": 16.1, "fft_y_fwd": 16.0, "solve_trid": 38.7,
+    "fp64": {"rk1": 51.2, "rk2_rhs": 87.7, "fft_x_fwd": 16.1, "fft_y_fwd": 16.0, "solve_trid": 38.7,
              "fft_y_bwd": 16.0, "fft_x_bwd": 16.0, "correct_vel": 60.9},
-    "fp32_mixed": {"rk1": 25.1, "rk2": 36.9, "fft_x_fwd_rhs": 20.9, "fft_y_fwd": 8.0,
-                   "solve_trid": 22.6, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
+    "fp32_mixed": {"rk1": 27.0, "rk2": 38.9, "fft_x_fwd_rhs": 21.0, "fft_y_fwd": 8.0,
+                   "solve_trid": 22.7, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
 }
 
 
@@ -152,6 +159,7 @@ def measure(wl, precision, steps, warmup):
     div = ch.calcMaxDiv()[0]
     ch.close()
     scale = 1.0 if precision == "fp64" else 0.5
+    table = BYTES_PER_POINT[precision]
     kernels = {}
     total = sum(ms for ms, n in times.values() if n)
     for name, (ms, n) in times.items():
@@ -159,15 +167,15 @@ def measure(wl, precision, steps, warmup):
             continue
         avg_ms = ms / n
         entry = {"avg_us": round(avg_ms * 1e3, 2), "launches": n, "share": round(ms / total, 4)}
-        bpp = BYTES_PER_POINT_FP64.get(name)
+        bpp = table.get(name)
         if bpp is not None:
-            gbs = bpp * scale * npts / (avg_ms * 1e-3) / 1e9
+            gbs = bpp * npts / (avg_ms * 1e-3) / 1e9
             entry["GBps"] = round(gbs, 1)
             entry["frac"] = round(gbs / HBM_PEAK_GBS, 4)
         kernels[name] = entry
     dom = max((k for k in kernels if "GBps" in kernels[k]), key=lambda k: times[k][0])
     d_avg_s = times[dom][0] / times[dom][1] * 1e-3
-    achieved = BYTES_PER_POINT_FP64[dom] * scale * npts / d_avg_s / 1e9
+    achieved = table[dom] * npts / d_avg_s / 1e9
     measured = MEASURED_HBM_B_PER_POINT.get(precision, {}).get(dom)
     traffic = (round(measured * npts) if measured is not None and wl == DEFAULT_WORKLOAD
                else None)
