@@ -22,7 +22,8 @@ __all__ = ["Channel", "Decomp", "PllError", "load_library", "mesh_by_func", "eig
            "PRECISIONS", "PROFILES", "LIB_PATH", "HEADER_PATH"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpll.so")
+# PLL_LIBRARY selects another build of the same C-ABI (e.g. to compare two builds)
+LIB_PATH = os.environ.get("PLL_LIBRARY") or os.path.join(_HERE, "libpll.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "pll.h")
 
 FIELDS = {"u": 0, "v": 1, "w": 2, "u1": 3, "v1": 4, "w1": 5, "p": 6}
