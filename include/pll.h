@@ -76,7 +76,8 @@ typedef struct pll_config {
   int precision;           /* pll_precision                                      */
   int pin_singular;        /* 1: the kx=ky=0 Neumann pivot is exactly zero, so the
                               solve takes the reference's b==0 branch
-                              (src/mod_poisson_solver.cc:341-346)              */
+                              (src/mod_poisson_solver.cc:341-346); PLL_FP32_MIXED
+                              always pins (p cannot carry the round-off constant) */
   int device;              /* HIP device ordinal                                 */
 } pll_config;
 

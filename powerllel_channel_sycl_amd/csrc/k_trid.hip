@@ -171,7 +171,10 @@ int trid_setup(pll_context* c) {
         col[k * P] -= (a_tmp * cc[k - 1]);
       }
     }
-  if (c->cfg.pin_singular) piv[(size_t)(nz - 1) * P] = 0.0;
+  // FP32-mixed always pins: the reference's round-off pivot would divide the
+  // FP32 round-off of the mean right-hand side into a constant of ~1e6 in p,
+  // which an FP32 p cannot carry next to O(1) pressure differences
+  if (c->cfg.pin_singular || c->prec != PLL_FP64) piv[(size_t)(nz - 1) * P] = 0.0;
   // FP32-mixed multiplies by reciprocal pivots (0 stands for a zero pivot:
   // the reference's b == 0 branch gives 0, src/mod_poisson_solver.cc:341-346)
   if (c->prec != PLL_FP64)
