@@ -197,6 +197,27 @@ int pll_stat_upload(pll_context *ctx, int var, const double *host);
 /* host[k-1] = sum over plane k of the sum array of var (k = 1..nz). */
 int pll_stat_plane_sums(pll_context *ctx, int var, double *host);
 
+/* ---- learned closure ------------------------------------------------------------ */
+/* No counterpart in the reference.  A 10 -> 64 -> 64 -> 1 perceptron (tanh
+ * hidden layers, softplus output) with BF16 weights and FP32 accumulation on
+ * MFMA, predicting a normalised eddy viscosity from 10 normalised features
+ * (9 filtered velocity gradients, wall distance).  Parameters (FP32 master):
+ * W1[16][64] (rows 10..15 zero), b1[64], W2[64][64], b2[64], W3[64], b3.
+ * Sample arrays are device pointers: x[n][10], y[n], out[n] (FP32). */
+typedef struct pll_closure pll_closure;
+int pll_closure_create(uint64_t seed, pll_closure **out);
+int pll_closure_destroy(pll_closure *c);
+int pll_closure_num_params(void);
+int pll_closure_get_params(pll_closure *c, float *host);
+int pll_closure_set_params(pll_closure *c, const float *host);
+/* Gradient of the mean squared error of softplus(o) against y (host copy), and the loss. */
+int pll_closure_gradient(pll_closure *c, const float *x, const float *y, long n, float *grad_host,
+                         double *loss);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) on the full batch. */
+int pll_closure_train_step(pll_closure *c, const float *x, const float *y, long n, double lr);
+int pll_closure_predict(pll_closure *c, const float *x, long n, float *out);
+int pll_closure_synchronize(pll_closure *c);
+
 /* ---- decomp_2d ------------------------------------------------------------------ */
 /* The 2D pencil decomposition for p_row x p_col ranks whose pencils all live
  * on this GPU (decomp_2d_init + decomp_info_init, src/decomp_2d.cc:41-207).

@@ -18,7 +18,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Channel", "Decomp", "PllError", "load_library", "mesh_by_func", "eigen_values", "FIELDS",
+__all__ = ["Channel", "Closure", "Decomp", "PllError", "load_library", "mesh_by_func", "eigen_values", "FIELDS",
            "PRECISIONS", "PROFILES", "LIB_PATH", "HEADER_PATH"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -101,6 +101,17 @@ _SIGNATURES = {
     "pll_stat_download": (C.c_int, [C.c_void_p, C.c_int, _D]),
     "pll_stat_upload": (C.c_int, [C.c_void_p, C.c_int, _D]),
     "pll_stat_plane_sums": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "pll_closure_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pll_closure_destroy": (C.c_int, [C.c_void_p]),
+    "pll_closure_num_params": (C.c_int, []),
+    "pll_closure_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pll_closure_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pll_closure_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                       C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+    "pll_closure_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                         C.c_double]),
+    "pll_closure_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]),
+    "pll_closure_synchronize": (C.c_int, [C.c_void_p]),
     "pll_decomp_create": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_void_p)]),
     "pll_decomp_destroy": (C.c_int, [C.c_void_p]),
     "pll_decomp_pencil": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
@@ -156,6 +167,59 @@ def eigen_values(n):
     out = np.zeros(n)
     _check(load_library().pll_eigen_values(n, out), "pll_eigen_values")
     return out
+
+
+class Closure:
+    """The learned eddy-viscosity closure (no counterpart in the reference): a
+    10 -> 64 -> 64 -> 1 perceptron with BF16 weights on MFMA.  Sample arrays
+    are device addresses of float32 buffers: x[n][10], y[n], out[n]."""
+
+    def __init__(self, seed=0):
+        self._h = C.c_void_p()
+        _check(load_library().pll_closure_create(int(seed), C.byref(self._h)), "pll_closure_create")
+        self.nparam = load_library().pll_closure_num_params()
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load_library().pll_closure_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.nparam, dtype=np.float32)
+        _check(load_library().pll_closure_get_params(
+            self._h, out.ctypes.data_as(C.POINTER(C.c_float))), "pll_closure_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        _check(load_library().pll_closure_set_params(
+            self._h, p.ctypes.data_as(C.POINTER(C.c_float))), "pll_closure_set_params")
+
+    def gradient(self, x_ptr, y_ptr, n):
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_double()
+        _check(load_library().pll_closure_gradient(
+            self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(n),
+            g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)), "pll_closure_gradient")
+        return g, loss.value
+
+    def train_step(self, x_ptr, y_ptr, n, lr):
+        _check(load_library().pll_closure_train_step(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                                     int(n), float(lr)), "pll_closure_train_step")
+
+    def predict(self, x_ptr, n, out_ptr):
+        _check(load_library().pll_closure_predict(self._h, C.c_void_p(x_ptr), int(n),
+                                                  C.c_void_p(out_ptr)), "pll_closure_predict")
+
+    def synchronize(self):
+        _check(load_library().pll_closure_synchronize(self._h), "pll_closure_synchronize")
 
 
 class Decomp:

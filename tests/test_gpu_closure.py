@@ -1,0 +1,100 @@
+"""The closure perceptron's MFMA kernels (csrc/k_closure.hip) through the
+C-ABI against the CPU restatement (oracle/pll_closure_oracle.py) with the same
+BF16 rounding points: inference, the gradient of every parameter block, and
+an Adam run that fits a smooth target."""
+import numpy as np
+import pytest
+
+import powerllel_channel_sycl_amd as pll
+from oracle import pll_closure_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1, (n, 10)).astype(np.float32)
+    t = 0.5 * x[:, 0] - 0.4 * x[:, 1] * x[:, 2] + 0.3 * np.sin(2 * x[:, 3]) + 0.2 * x[:, 9]
+    y = np.log1p(np.exp(t)).astype(np.float32)
+    return x, y
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def _close_to_oracle(got, want):
+    """GPU tanhf and the oracle's tanh may differ by an FP32 ulp; where that lands
+    on a BF16 rounding boundary of h1 or h2, one activation moves by a BF16 step.
+    So: nearly all outputs to 1e-4, the rare rest within a few BF16 steps."""
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3)
+    assert np.mean(err > 1e-4) < 0.01, np.mean(err > 1e-4)
+    assert np.max(err) < 1e-2, np.max(err)
+
+
+def _random_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, scale, mo.N_PARAM).astype(np.float32)
+    p[mo.O_W1:mo.O_B1].reshape(16, 64)[10:] = 0.0
+    return p
+
+
+@pytest.mark.parametrize("n", [32, 1000, 4096])
+def test_predict_matches_oracle(n):
+    import torch
+    print(f"[closure] predict n={n}", flush=True)
+    cl = pll.Closure(seed=1)
+    p = _random_params(5)
+    cl.set_params(p)
+    np.testing.assert_array_equal(cl.params(), p)
+    x, _ = _data(n, 2)
+    dx = _dev(x)
+    out = torch.full((n,), float("nan"), device="cuda:0")
+    cl.predict(dx.data_ptr(), n, out.data_ptr())
+    cl.synchronize()
+    _close_to_oracle(out.cpu().numpy(), mo.predict(p, x))
+    cl.close()
+
+
+@pytest.mark.parametrize("n", [100, 2500])
+def test_gradient_matches_oracle(n):
+    print(f"[closure] gradient n={n}", flush=True)
+    cl = pll.Closure(seed=1)
+    p = _random_params(6)
+    cl.set_params(p)
+    x, y = _data(n, 3)
+    dx, dy = _dev(x), _dev(y)  # held: a temporary's memory could be reused at once
+    g, loss = cl.gradient(dx.data_ptr(), dy.data_ptr(), n)
+    gr, lr_ = mo.gradient(p, x, y)
+    assert abs(loss - lr_) <= 1e-4 * abs(lr_)
+    blocks = {"W1": (mo.O_W1, mo.O_B1), "b1": (mo.O_B1, mo.O_W2), "W2": (mo.O_W2, mo.O_B2),
+              "b2": (mo.O_B2, mo.O_W3), "W3": (mo.O_W3, mo.O_B3), "b3": (mo.O_B3, mo.N_PARAM)}
+    for name, (a, b) in blocks.items():
+        err = np.linalg.norm(g[a:b] - gr[a:b]) / max(np.linalg.norm(gr[a:b]), 1e-30)
+        print(f"  {name}: relative error {err:.2e}")
+        assert err < 2e-3, (name, err)
+    cl.close()
+
+
+def test_adam_fits_a_smooth_target():
+    print("[closure] training", flush=True)
+    cl = pll.Closure(seed=7)
+    x, y = _data(8192, 4)
+    xt, yt = _data(2048, 5)
+    dx, dy, dxt = _dev(x), _dev(y), _dev(xt)
+    _, l0 = cl.gradient(dx.data_ptr(), dy.data_ptr(), len(y))
+    for _ in range(400):
+        cl.train_step(dx.data_ptr(), dy.data_ptr(), len(y), 3e-3)
+    _, l1 = cl.gradient(dx.data_ptr(), dy.data_ptr(), len(y))
+    import torch
+    out = torch.zeros(len(yt), device="cuda:0")
+    cl.predict(dxt.data_ptr(), len(yt), out.data_ptr())
+    cl.synchronize()
+    test_mse = float(np.mean((out.cpu().numpy() - yt) ** 2))
+    print(f"  loss {l0:.4f} -> {l1:.4f}; held-out mse {test_mse:.4f} (target variance {np.var(yt):.4f})")
+    assert l1 < 0.1 * l0
+    assert test_mse < 0.2 * np.var(yt)
+    # the trained network agrees with the oracle evaluated on its parameters
+    _close_to_oracle(out.cpu().numpy(), mo.predict(cl.params(), xt))
+    cl.close()
