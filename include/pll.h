@@ -217,6 +217,36 @@ int pll_closure_gradient(pll_closure *c, const float *x, const float *y, long n,
 int pll_closure_train_step(pll_closure *c, const float *x, const float *y, long n, double lr);
 int pll_closure_predict(pll_closure *c, const float *x, long n, float *out);
 int pll_closure_synchronize(pll_closure *c);
+/* Scales of the network's inputs and output: features are the 9 velocity
+ * gradients at a cell centre times gscale (du/dz, dv/dz, dw/dx, dw/dy negated
+ * in the upper half channel, so both walls look alike) and the distance to the
+ * nearer wall times yscale; the eddy viscosity is nuscale * softplus(o). */
+int pll_closure_set_scales(pll_closure *c, double gscale, double yscale, double nuscale);
+int pll_closure_get_scales(pll_closure *c, double *gscale, double *yscale, double *nuscale);
+/* Attach a closure to a context (NULL detaches; detach before destroying the
+ * closure).  With a closure attached, timeIntVelRK1/RK2 and pll_step add the
+ * divergence of the eddy-viscosity stress 2 nu_t S_ij to the momentum
+ * tendency (src/mod_calcVel.cc:27-242), with nu_t evaluated by the network
+ * inside the stage kernel; pll_step then runs the unfused sequence
+ * src/main.cc:117-153.  Training the closure while attached is ordered
+ * before the context's next stage. */
+int pll_closure_attach(pll_context *ctx, pll_closure *c);
+/* nu_t of the context's last closure stage, interior nz*ny*nx, x fastest. */
+int pll_closure_nut(pll_context *ctx, double *host);
+/* Box filter of the fine context's u, v, w onto the coarse context's grid
+ * (factors nx_f/nx_c, ny_f/ny_c, nz_f/nz_c; every coarse z cell the union of
+ * fine ones): each coarse face value is the area average of the fine face
+ * values on that face, so divergence-free stays divergence-free; the coarse
+ * halos are refreshed.  Blocks until done. */
+int pll_closure_filter(pll_context *fine, pll_context *coarse);
+/* Training samples of the fine context's current state: filters it into the
+ * coarse context, then for each of the n = nx_c*ny_c*nz_c coarse cells writes
+ * the features (x[n][10], device) and the label (y[n], device): the
+ * least-squares eddy viscosity of the subgrid stress of the box
+ * max(0, -tau^d_ij S_ij) / (2 S_ij S_ij + 1e-4/gscale^2) / nuscale, at most 50.
+ * Blocks until done. */
+int pll_closure_samples(pll_context *fine, pll_context *coarse, pll_closure *c, float *x,
+                        float *y);
 
 /* ---- decomp_2d ------------------------------------------------------------------ */
 /* The 2D pencil decomposition for p_row x p_col ranks whose pencils all live

@@ -112,6 +112,13 @@ _SIGNATURES = {
                                          C.c_double]),
     "pll_closure_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]),
     "pll_closure_synchronize": (C.c_int, [C.c_void_p]),
+    "pll_closure_set_scales": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "pll_closure_get_scales": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
+    "pll_closure_attach": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pll_closure_nut": (C.c_int, [C.c_void_p, _D]),
+    "pll_closure_filter": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pll_closure_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pll_decomp_create": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_void_p)]),
     "pll_decomp_destroy": (C.c_int, [C.c_void_p]),
     "pll_decomp_pencil": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
@@ -220,6 +227,22 @@ class Closure:
 
     def synchronize(self):
         _check(load_library().pll_closure_synchronize(self._h), "pll_closure_synchronize")
+
+    def set_scales(self, gscale, yscale, nuscale):
+        """Inputs: gradients * gscale, wall distance * yscale; nu_t = nuscale * softplus(o)."""
+        _check(load_library().pll_closure_set_scales(self._h, float(gscale), float(yscale),
+                                                     float(nuscale)), "pll_closure_set_scales")
+
+    def scales(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(load_library().pll_closure_get_scales(self._h, C.byref(a), C.byref(b), C.byref(c)),
+               "pll_closure_get_scales")
+        return a.value, b.value, c.value
+
+    def samples(self, fine, coarse, x_ptr, y_ptr):
+        """Filter fine into coarse and write its n = coarse cells training samples."""
+        _check(load_library().pll_closure_samples(fine._h, coarse._h, self._h, C.c_void_p(x_ptr),
+                                                  C.c_void_p(y_ptr)), "pll_closure_samples")
 
 
 class Decomp:
@@ -445,6 +468,21 @@ class Channel:
         ok = C.c_int()
         self._call("pll_check_nan", FIELDS[field], C.byref(ok))
         return bool(ok.value)
+
+    # ---- learned closure --------------------------------------------------------------
+    def attach_closure(self, closure):
+        """Add the closure's eddy-viscosity stress to the RK stages (None detaches)."""
+        self._call("pll_closure_attach", closure._h if closure is not None else None)
+        self._closure = closure  # keep it alive while attached
+
+    def closure_nut(self):
+        out = np.zeros((self.nz, self.ny, self.nx))
+        self._call("pll_closure_nut", out)
+        return out
+
+    def filter_from(self, fine):
+        """This (coarse) context's u, v, w := the box filter of the fine context's."""
+        _check(load_library().pll_closure_filter(fine._h, self._h), "pll_closure_filter")
 
     # ---- main loop ---------------------------------------------------------------------
     def step(self, nsteps=1):

@@ -16,11 +16,14 @@
 // Training is deterministic: each wave accumulates its gradient over its
 // tiles in registers, writes one partial per wave, and a second kernel sums
 // the partials in a fixed order before the Adam update.
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <random>
 #include <vector>
 
 #include "pll_internal.hpp"
+#include "stencil_ops.hpp"
 
 namespace pll {
 namespace {
@@ -31,8 +34,8 @@ constexpr int kH = 64;    // hidden width
 // parameter vector (FP32 master): W1[16][64], b1[64], W2[64][64], b2[64], W3[64], b3
 constexpr int oW1 = 0, oB1 = oW1 + kInP * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH,
               oW3 = oB2 + kH, oB3 = oW3 + kH, kNParam = oB3 + 1;
-constexpr int kWaves = 4;  // waves per block
-constexpr int kTilesPerWave = 4;
+constexpr int kWaves = 4;          // waves per block
+constexpr long kMaxWaves = 4096;   // training grid (partials are kMaxWaves x kNParam)
 
 typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
@@ -82,17 +85,23 @@ __device__ inline bf16x8 load8(const uint16_t* p) {
 // D-layout row of accumulator element j for this lane.
 __device__ inline int drow(int j, int lane) { return 8 * (j / 4) + 4 * (lane / 32) + (j % 4); }
 
-// Forward pass of one 32-sample tile.  x: features of samples m0..m0+31 (rows
-// beyond n are zero); fills L.x/h1/h2 and returns z1/z2 activations in D layout.
-__device__ inline void forward_tile(const Mlp& w, const float* __restrict__ x, long m0, long n,
-                                    WaveLds& L, int lane, f32x16 h1[2], f32x16 h2[2],
-                                    float& out_o) {
-  // stage X (BF16, 10 features + 6 zeros)
+// X of one 32-sample tile from a feature array (rows beyond n are zero).
+template <class L_>
+__device__ inline void stage_x(const float* __restrict__ x, long m0, long n, L_& L, int lane) {
   for (int e = lane; e < 32 * kInP; e += 64) {
     const int m = e / kInP, k = e % kInP;
     const long s = m0 + m;
     L.x[m][k] = to_bf16((k < kIn && s < n) ? x[s * kIn + k] : 0.f);
   }
+}
+
+// Forward pass of one 32-sample tile whose BF16 features are in L.x (written
+// before the call by any lane of the wave): fills L.h1/h2, returns the tanh
+// activations in D layout and, on lanes 0..31, the output o of sample lane.
+// Every wave of the block calls it the same number of times (it holds barriers).
+template <class L_>
+__device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2], f32x16 h2[2],
+                                    float& out_o) {
   __syncthreads();
   const int col = lane % 32, kh = 8 * (lane / 32);
   // layer 1: h1 = tanh(X W1 + b1)
@@ -151,7 +160,8 @@ template <bool TRAIN>
 __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restrict__ x,
                                                       const float* __restrict__ y, long n,
                                                       float* __restrict__ out,
-                                                      float* __restrict__ partial) {
+                                                      float* __restrict__ partial,
+                                                      int tiles_per_wave) {
   __shared__ WaveLds lds[kWaves];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   WaveLds& L = lds[wave];
@@ -161,12 +171,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
   f32x16 gW1[2] = {};     // dW1 tiles [n-tile] (rows i < 16 meaningful)
   float gb1[2] = {0.f, 0.f}, gb2[2] = {0.f, 0.f}, gw3[2] = {0.f, 0.f}, gb3 = 0.f, loss = 0.f;
   const float inv_n = 1.f / (float)n;
-  for (int tile = 0; tile < kTilesPerWave; ++tile) {
-    const long m0 = (gw * kTilesPerWave + tile) * 32;
+  const long nwaves = (long)gridDim.x * kWaves;
+  for (int tile = 0; tile < tiles_per_wave; ++tile) {
+    const long m0 = (gw + tile * nwaves) * 32;
     // every wave of a block runs the same number of tiles (barriers inside)
     f32x16 h1[2], h2[2];
     float o;
-    forward_tile(w, x, m0, n, L, lane, h1, h2, o);
+    stage_x(x, m0, n, L, lane);
+    forward_tile(w, L, lane, h1, h2, o);
     if (!TRAIN) {
       if (lane < 32 && m0 + lane < n) out[m0 + lane] = softplus(o);
       __syncthreads();
@@ -336,6 +348,312 @@ __global__ void k_refresh_bf16(const float* __restrict__ p, uint16_t* __restrict
   else if (i >= oW3 && i < oB3) w3b[i - oW3] = to_bf16(p[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Features, training samples and the closure stage.
+//
+// The nine velocity gradients at the centre of cell (i,j,k) of a staggered
+// field (u at i+1/2, v at j+1/2, w at k+1/2), from its halo-padded arrays:
+// g = {du/dx, du/dy, du/dz, dv/dx, dv/dy, dv/dz, dw/dx, dw/dy, dw/dz}.  Normal
+// derivatives are the face differences across the cell; cross derivatives are
+// central differences of the cell-centred averages of the neighbours
+// (qdxi = dxi/4, qdyi = dyi/4, hz = 1/(2 (zc(k+1) - zc(k-1)))).  a = idx(i,j,k)
+// with i, j interior and 1 <= k <= nz.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void cell_gradient(const T* __restrict__ u, const T* __restrict__ v,
+                                     const T* __restrict__ w, const Geom& g, long a, T dxi,
+                                     T dyi, T qdxi, T qdyi, T dfi, T hz, T gr[9]) {
+  const long sx = g.sx, sz = g.sxy;
+  gr[0] = (u[a] - u[a - 1]) * dxi;
+  gr[1] = ((u[a + sx] + u[a - 1 + sx]) - (u[a - sx] + u[a - 1 - sx])) * qdyi;
+  gr[2] = ((u[a + sz] + u[a - 1 + sz]) - (u[a - sz] + u[a - 1 - sz])) * hz;
+  gr[3] = ((v[a + 1] + v[a + 1 - sx]) - (v[a - 1] + v[a - 1 - sx])) * qdxi;
+  gr[4] = (v[a] - v[a - sx]) * dyi;
+  gr[5] = ((v[a + sz] + v[a - sx + sz]) - (v[a - sz] + v[a - sx - sz])) * hz;
+  gr[6] = ((w[a + 1] + w[a + 1 - sz]) - (w[a - 1] + w[a - 1 - sz])) * qdxi;
+  gr[7] = ((w[a + sx] + w[a + sx - sz]) - (w[a - sx] + w[a - sx - sz])) * qdyi;
+  gr[8] = (w[a] - w[a - sz]) * dfi;
+}
+
+// Network inputs: the gradients times gs, with the four components that
+// change sign under the reflection z -> lz - z (du/dz, dv/dz, dw/dx, dw/dy)
+// multiplied by zs = -1 in the upper half, so both walls look like the lower
+// one; the distance to the nearer wall times ys.
+template <typename T>
+__device__ inline void features(const T gr[9], T gs, T ys, T yw, T zs, float f[kIn]) {
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    T q = gr[c] * gs;
+    if (c == 2 || c == 5 || c == 6 || c == 7) q = q * zs;
+    f[c] = (float)q;
+  }
+  f[9] = (float)(yw * ys);
+}
+
+// Per-plane coefficients of the features (length nz+2 each): hz, the wall
+// distance of the cell centre and the reflection sign.
+template <typename T>
+struct FeatMesh {
+  const T* hz;
+  const T* yw;
+  const T* zs;
+};
+
+// ---- box filter: the coarse staggered field whose every face value is the
+// area average of the fine face values on it (fine faces nest in coarse ones:
+// nx_f = fx nx_c, ny_f = fy ny_c, nz_f = fz nz_c with nesting z faces).  The
+// coarse flux through each face is the sum of the fine fluxes, so a discretely
+// divergence-free fine field gives a divergence-free coarse one.
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(256) void k_filter(Geom gf, const Tf* __restrict__ uf,
+                                                const Tf* __restrict__ vf,
+                                                const Tf* __restrict__ wf,
+                                                const double* __restrict__ dzf_f, Geom gc,
+                                                Tc* __restrict__ uc, Tc* __restrict__ vc,
+                                                Tc* __restrict__ wc, int fx, int fy, int fz) {
+  const long n = (long)blockIdx.x * 256 + threadIdx.x;
+  const long ncell = (long)gc.nx * gc.ny * gc.nz;
+  if (n >= ncell) return;
+  const int I = (int)(n % gc.nx) + 1, J = (int)((n / gc.nx) % gc.ny) + 1,
+            K = (int)(n / ((long)gc.nx * gc.ny)) + 1;
+  const int ib = (I - 1) * fx, jb = (J - 1) * fy, kb = (K - 1) * fz;  // fine cells ib+1..ib+fx
+  double su = 0.0, sv = 0.0, dzs = 0.0;
+  for (int kk = 1; kk <= fz; ++kk) {
+    const double dz = dzf_f[kb + kk];
+    double tu = 0.0, tv = 0.0;
+    for (int jj = 1; jj <= fy; ++jj) tu += (double)uf[gf.idx(I * fx, jb + jj, kb + kk)];
+    for (int ii = 1; ii <= fx; ++ii) tv += (double)vf[gf.idx(ib + ii, J * fy, kb + kk)];
+    su += tu * dz;
+    sv += tv * dz;
+    dzs += dz;
+  }
+  double sw = 0.0;
+  for (int jj = 1; jj <= fy; ++jj)
+    for (int ii = 1; ii <= fx; ++ii) sw += (double)wf[gf.idx(ib + ii, jb + jj, K * fz)];
+  const long a = gc.idx(I, J, K);
+  uc[a] = (Tc)(su / (fy * dzs));
+  vc[a] = (Tc)(sv / (fx * dzs));
+  wc[a] = (Tc)(sw / (fx * fy));
+}
+
+// ---- training samples: for every coarse cell, the features of the filtered
+// field (the coarse context, which holds the filter of the fine one with its
+// halos) and the label: the eddy viscosity nu* that best fits the subgrid
+// stress of the fine field by least squares,
+//   tau_ij = <uc_i uc_j> - <uc_i><uc_j>   (<.> the dz-weighted box average of
+//                                          fine cell-centre velocities),
+//   nu* = max(0, -tau^d_ij S_ij) / (2 S_ij S_ij + eps),
+// with S the strain rate of the filtered field at the cell centre; the label
+// is min(nu* / nus, ymax).
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(256) void k_samples(
+    Geom gf, const Tf* __restrict__ uf, const Tf* __restrict__ vf, const Tf* __restrict__ wf,
+    const double* __restrict__ dzf_f, Geom gc, const Tc* __restrict__ uc,
+    const Tc* __restrict__ vc, const Tc* __restrict__ wc, const Tc* __restrict__ dzf_inv_c,
+    FeatMesh<Tc> fm, Tc dxi, Tc dyi, Tc qdxi, Tc qdyi, Tc gs, Tc ys, double nus, double eps,
+    double ymax, int fx, int fy, int fz, float* __restrict__ x, float* __restrict__ y) {
+  const long n = (long)blockIdx.x * 256 + threadIdx.x;
+  const long ncell = (long)gc.nx * gc.ny * gc.nz;
+  if (n >= ncell) return;
+  const int I = (int)(n % gc.nx) + 1, J = (int)((n / gc.nx) % gc.ny) + 1,
+            K = (int)(n / ((long)gc.nx * gc.ny)) + 1;
+  Tc gr[9];
+  cell_gradient(uc, vc, wc, gc, gc.idx(I, J, K), dxi, dyi, qdxi, qdyi, dzf_inv_c[K], fm.hz[K], gr);
+  float f[kIn];
+  features(gr, gs, ys, fm.yw[K], fm.zs[K], f);
+#pragma unroll
+  for (int c = 0; c < kIn; ++c) x[n * kIn + c] = f[c];
+  // box moments of the fine cell-centre velocities
+  const int ib = (I - 1) * fx, jb = (J - 1) * fy, kb = (K - 1) * fz;
+  double m[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0}, wsum = 0.0;
+  for (int kk = 1; kk <= fz; ++kk) {
+    const double dz = dzf_f[kb + kk];
+    double pm[3] = {0, 0, 0}, pq[6] = {0, 0, 0, 0, 0, 0};
+    for (int jj = 1; jj <= fy; ++jj)
+      for (int ii = 1; ii <= fx; ++ii) {
+        const long a = gf.idx(ib + ii, jb + jj, kb + kk);
+        const double cu = 0.5 * ((double)uf[a - 1] + (double)uf[a]);
+        const double cv = 0.5 * ((double)vf[a - gf.sx] + (double)vf[a]);
+        const double cw = 0.5 * ((double)wf[a - gf.sxy] + (double)wf[a]);
+        pm[0] += cu;
+        pm[1] += cv;
+        pm[2] += cw;
+        pq[0] += cu * cu;
+        pq[1] += cv * cv;
+        pq[2] += cw * cw;
+        pq[3] += cu * cv;
+        pq[4] += cu * cw;
+        pq[5] += cv * cw;
+      }
+    for (int c = 0; c < 3; ++c) m[c] += pm[c] * dz;
+    for (int c = 0; c < 6; ++c) q[c] += pq[c] * dz;
+    wsum += dz * (fx * fy);
+  }
+  for (int c = 0; c < 3; ++c) m[c] /= wsum;
+  for (int c = 0; c < 6; ++c) q[c] /= wsum;
+  const double txx = q[0] - m[0] * m[0], tyy = q[1] - m[1] * m[1], tzz = q[2] - m[2] * m[2];
+  const double txy = q[3] - m[0] * m[1], txz = q[4] - m[0] * m[2], tyz = q[5] - m[1] * m[2];
+  const double sxx = gr[0], syy = gr[4], szz = gr[8];
+  const double sxy = 0.5 * ((double)gr[1] + (double)gr[3]);
+  const double sxz = 0.5 * ((double)gr[2] + (double)gr[6]);
+  const double syz = 0.5 * ((double)gr[5] + (double)gr[7]);
+  const double ss = ((sxx * sxx + syy * syy) + szz * szz) + 2.0 * ((sxy * sxy + sxz * sxz) + syz * syz);
+  const double ts = ((txx * sxx + tyy * syy) + tzz * szz) + 2.0 * ((txy * sxy + txz * sxz) + tyz * syz) -
+                    ((txx + tyy) + tzz) / 3.0 * ((sxx + syy) + szz);
+  const double nu = fmax(0.0, -ts) / (2.0 * ss + eps);
+  y[n] = (float)fmin(nu / nus, ymax);
+}
+
+// ---- the closure stage: RK stage (src/mod_calcVel.cc:27-242 in the
+// reference's arithmetic) plus the divergence of the eddy-viscosity stress
+// 2 nu_t S_ij, with nu_t = nus * softplus(MLP(features)) evaluated in the same
+// kernel.  A block owns 32 x 8 x 4 cells: phase 1 runs the network on the
+// 34 x 10 x 6 cells around them (periodic images in x and y; 0 outside
+// 1 <= k <= nz) into LDS, phase 2 updates the velocities.  Staggered stress
+// divergence: normal stresses at cell centres, shear stresses at cell edges
+// with nu_t averaged from the four cells around the edge; the wall edges carry
+// no eddy viscosity (the wall stress is the molecular one).
+constexpr int kLx = 32, kLy = 8, kLz = 4;
+constexpr int kEx = kLx + 2, kEy = kLy + 2, kEz = kLz + 2, kExt = kEx * kEy * kEz;
+constexpr int kLesTiles = (kExt + 31) / 32;
+static_assert(kLesTiles % kWaves == 0, "every wave runs the same number of network tiles");
+static_assert(kLx * kLy == 64 * kWaves, "one thread per (i, j) column of the tile");
+
+struct FwdLds {
+  uint16_t x[32][kInP];
+  uint16_t h1[32][kH];
+  uint16_t h2[32][kH];
+};
+
+template <typename T>
+struct GAcc {
+  const T* p;
+  long sx, sxy;
+  __device__ __forceinline__ T operator()(int di, int dj, int dk) const {
+    return p[di + dj * sx + dk * sxy];
+  }
+};
+
+template <typename T, int STAGE>
+__global__ __launch_bounds__(64 * kWaves) void k_rk_les(
+    Geom g, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
+    T* __restrict__ du, T* __restrict__ dv, T* __restrict__ dw, MeshT<T> m, FeatMesh<T> fm,
+    T dxi, T dyi, T qdxi, T qdyi, T rei, T dt, T ucrf, Mlp w, T gs, T ys, float nus,
+    float* __restrict__ nut, int ntx, int nty) {
+  __shared__ FwdLds lds[kWaves];
+  __shared__ float nu[kExt];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int b = blockIdx.x;
+  const int ti = b % ntx, tj = (b / ntx) % nty, tk = b / (ntx * nty);
+  const int i0 = ti * kLx, j0 = tj * kLy, k0 = tk * kLz;  // extended cell (0,0,0) = (i0,j0,k0)
+  FwdLds& L = lds[wave];
+  // phase 1: eddy viscosity of the extended block
+  for (int t = wave; t < kLesTiles; t += kWaves) {
+    const int e = t * 32 + lane % 32;
+    bool valid = false;
+    if (lane < 32) {
+      float f[kInP] = {};
+      if (e < kExt) {
+        const int ex = e % kEx, ey = (e / kEx) % kEy, ez = e / (kEx * kEy);
+        int i = i0 + ex, j = j0 + ey;
+        const int k = k0 + ez;
+        valid = i <= g.nx + 1 && j <= g.ny + 1 && k >= 1 && k <= g.nz;
+        if (valid) {
+          i = i == 0 ? g.nx : (i == g.nx + 1 ? 1 : i);
+          j = j == 0 ? g.ny : (j == g.ny + 1 ? 1 : j);
+          T gr[9];
+          cell_gradient(su, sv, sw, g, g.idx(i, j, k), dxi, dyi, qdxi, qdyi, m.dzf_inv[k],
+                        fm.hz[k], gr);
+          features(gr, gs, ys, fm.yw[k], fm.zs[k], f);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kInP; ++q) L.x[lane][q] = to_bf16(f[q]);
+    }
+    f32x16 h1[2], h2[2];
+    float o;
+    forward_tile(w, L, lane, h1, h2, o);
+    if (lane < 32 && e < kExt) nu[e] = valid ? softplus(o) * nus : 0.f;
+  }
+  __syncthreads();
+  // phase 2: the stage update of the block's cells
+  const int tx = threadIdx.x % kLx, ty = threadIdx.x / kLx;
+  const int i = i0 + tx + 1, j = j0 + ty + 1;
+  if (i > g.nx || j > g.ny) return;
+  const T two = T(2), qtr = T(0.25);
+  for (int kz = 0; kz < kLz; ++kz) {
+    const int k = k0 + kz + 1;
+    if (k > g.nz) break;
+    const long a = g.idx(i, j, k);
+    const GAcc<T> U{su + a, g.sx, g.sxy}, V{sv + a, g.sx, g.sxy}, W{sw + a, g.sx, g.sxy};
+    T tu, tv, tw;
+    tendency<T>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw);
+    const int c = (tx + 1) + kEx * ((ty + 1) + kEy * (kz + 1));
+    auto N = [&](int di, int dj, int dk) -> T { return (T)nu[c + di + kEx * (dj + kEy * dk)]; };
+    const T dci = m.dzc_inv[k], dcm = m.dzc_inv[k - 1], dfi = m.dzf_inv[k];
+    const bool top = k == g.nz, bot = k == 1;
+    T s_u, s_v, s_w;
+    {  // x-momentum at the u face (i+1/2, j, k)
+      const T sE = (U(1, 0, 0) - U(0, 0, 0)) * dxi, sW = (U(0, 0, 0) - U(-1, 0, 0)) * dxi;
+      const T fxx = ((N(1, 0, 0) * sE - N(0, 0, 0) * sW) * two) * dxi;
+      const T nN = qtr * (((N(0, 0, 0) + N(1, 0, 0)) + N(0, 1, 0)) + N(1, 1, 0));
+      const T nS = qtr * (((N(0, -1, 0) + N(1, -1, 0)) + N(0, 0, 0)) + N(1, 0, 0));
+      const T gN = (U(0, 1, 0) - U(0, 0, 0)) * dyi + (V(1, 0, 0) - V(0, 0, 0)) * dxi;
+      const T gS = (U(0, 0, 0) - U(0, -1, 0)) * dyi + (V(1, -1, 0) - V(0, -1, 0)) * dxi;
+      const T fxy = (nN * gN - nS * gS) * dyi;
+      const T nT = top ? T(0) : qtr * (((N(0, 0, 0) + N(1, 0, 0)) + N(0, 0, 1)) + N(1, 0, 1));
+      const T nB = bot ? T(0) : qtr * (((N(0, 0, -1) + N(1, 0, -1)) + N(0, 0, 0)) + N(1, 0, 0));
+      const T gT = (U(0, 0, 1) - U(0, 0, 0)) * dci + (W(1, 0, 0) - W(0, 0, 0)) * dxi;
+      const T gB = (U(0, 0, 0) - U(0, 0, -1)) * dcm + (W(1, 0, -1) - W(0, 0, -1)) * dxi;
+      const T fxz = (nT * gT - nB * gB) * dfi;
+      s_u = (fxx + fxy) + fxz;
+    }
+    {  // y-momentum at the v face (i, j+1/2, k)
+      const T nE = qtr * (((N(0, 0, 0) + N(1, 0, 0)) + N(0, 1, 0)) + N(1, 1, 0));
+      const T nW = qtr * (((N(-1, 0, 0) + N(0, 0, 0)) + N(-1, 1, 0)) + N(0, 1, 0));
+      const T gE = (U(0, 1, 0) - U(0, 0, 0)) * dyi + (V(1, 0, 0) - V(0, 0, 0)) * dxi;
+      const T gW = (U(-1, 1, 0) - U(-1, 0, 0)) * dyi + (V(0, 0, 0) - V(-1, 0, 0)) * dxi;
+      const T fyx = (nE * gE - nW * gW) * dxi;
+      const T sN = (V(0, 1, 0) - V(0, 0, 0)) * dyi, sS = (V(0, 0, 0) - V(0, -1, 0)) * dyi;
+      const T fyy = ((N(0, 1, 0) * sN - N(0, 0, 0) * sS) * two) * dyi;
+      const T nT = top ? T(0) : qtr * (((N(0, 0, 0) + N(0, 1, 0)) + N(0, 0, 1)) + N(0, 1, 1));
+      const T nB = bot ? T(0) : qtr * (((N(0, 0, -1) + N(0, 1, -1)) + N(0, 0, 0)) + N(0, 1, 0));
+      const T gT = (V(0, 0, 1) - V(0, 0, 0)) * dci + (W(0, 1, 0) - W(0, 0, 0)) * dyi;
+      const T gB = (V(0, 0, 0) - V(0, 0, -1)) * dcm + (W(0, 1, -1) - W(0, 0, -1)) * dyi;
+      const T fyz = (nT * gT - nB * gB) * dfi;
+      s_v = (fyx + fyy) + fyz;
+    }
+    if (top) {
+      s_w = T(0);  // w(nz) is the wall value
+    } else {  // z-momentum at the w face (i, j, k+1/2)
+      const T nE = qtr * (((N(0, 0, 0) + N(1, 0, 0)) + N(0, 0, 1)) + N(1, 0, 1));
+      const T nW = qtr * (((N(-1, 0, 0) + N(0, 0, 0)) + N(-1, 0, 1)) + N(0, 0, 1));
+      const T gE = (U(0, 0, 1) - U(0, 0, 0)) * dci + (W(1, 0, 0) - W(0, 0, 0)) * dxi;
+      const T gW = (U(-1, 0, 1) - U(-1, 0, 0)) * dci + (W(0, 0, 0) - W(-1, 0, 0)) * dxi;
+      const T fzx = (nE * gE - nW * gW) * dxi;
+      const T nN = qtr * (((N(0, 0, 0) + N(0, 1, 0)) + N(0, 0, 1)) + N(0, 1, 1));
+      const T nS = qtr * (((N(0, -1, 0) + N(0, 0, 0)) + N(0, -1, 1)) + N(0, 0, 1));
+      const T gN = (V(0, 0, 1) - V(0, 0, 0)) * dci + (W(0, 1, 0) - W(0, 0, 0)) * dyi;
+      const T gS = (V(0, -1, 1) - V(0, -1, 0)) * dci + (W(0, 0, 0) - W(0, -1, 0)) * dyi;
+      const T fzy = (nN * gN - nS * gS) * dyi;
+      const T sT = (W(0, 0, 1) - W(0, 0, 0)) * m.dzf_inv[k + 1];
+      const T sB = (W(0, 0, 0) - W(0, 0, -1)) * dfi;
+      const T fzz = ((N(0, 0, 1) * sT - N(0, 0, 0) * sB) * two) * dci;
+      s_w = (fzx + fzy) + fzz;
+    }
+    if (STAGE == 1) {
+      du[a] = U(0, 0, 0) + dt * tu + dt * s_u;
+      dv[a] = V(0, 0, 0) + dt * tv + dt * s_v;
+      dw[a] = W(0, 0, 0) + dt * tw + dt * s_w;
+    } else {
+      du[a] = (U(0, 0, 0) + dt * tu + du[a] + dt * s_u) * T(0.5);
+      dv[a] = (V(0, 0, 0) + dt * tv + dv[a] + dt * s_v) * T(0.5);
+      dw[a] = (W(0, 0, 0) + dt * tw + dw[a] + dt * s_w) * T(0.5);
+    }
+    if (nut) nut[((long)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = nu[c];
+  }
+}
+
 }  // namespace
 }  // namespace pll
 
@@ -351,6 +669,9 @@ struct pll_closure {
   float* grad = nullptr;  // kNParam + 1 (loss sum last)
   long step = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t ready = nullptr;  // orders a context's closure stages after training
+  // feature and output scales: inputs g*gs and (wall distance)*ys, nu_t = nus*softplus(o)
+  double gs = 1.0, ys = 1.0, nus = 1.0;
 };
 
 namespace {
@@ -368,10 +689,17 @@ int ensure_partial(pll_closure* c, long nwaves) {
   return PLL_OK;
 }
 
+// Waves of a k_mlp grid for n samples (a multiple of kWaves, at most
+// kMaxWaves) and the 32-sample tiles each wave runs (wave q takes tiles q,
+// q + nwaves, ...).
 long waves_for(long n) {
   const long tiles = (n + 31) / 32;
-  const long waves = (tiles + kTilesPerWave - 1) / kTilesPerWave;
+  const long waves = std::min(tiles, kMaxWaves);
   return (waves + kWaves - 1) / kWaves * kWaves;
+}
+int tiles_per_wave(long n, long nwaves) {
+  const long tiles = (n + 31) / 32;
+  return (int)((tiles + nwaves - 1) / nwaves);
 }
 
 int gradient(pll_closure* c, const float* x, const float* y, long n) {
@@ -379,16 +707,227 @@ int gradient(pll_closure* c, const float* x, const float* y, long n) {
   int rc = ensure_partial(c, nw);
   if (rc) return rc;
   hipLaunchKernelGGL((k_mlp<true>), dim3((unsigned)(nw / kWaves)), dim3(64 * kWaves), 0, c->stream,
-                     weights(c), x, y, n, static_cast<float*>(nullptr), c->partial);
+                     weights(c), x, y, n, static_cast<float*>(nullptr), c->partial,
+                     tiles_per_wave(n, nw));
   hipLaunchKernelGGL(k_grad_reduce, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream,
                      c->partial, nw, c->grad);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
 
+// Label regularisation: eps of nu* in units of the squared scaled strain rate,
+// and the largest label.
+constexpr double kLabelEps = 1e-4;
+constexpr double kLabelMax = 50.0;
+
+template <typename T>
+int feat_setup(pll_context* c) {
+  if (c->d_feat) return PLL_OK;
+  const int nz = c->g.nz;
+  const double lz = c->cfg.lz;
+  std::vector<T> h(3 * (nz + 2), T(0));
+  for (int k = 1; k <= nz; ++k) {
+    const double zc = c->zc_global[k];
+    h[k] = (T)(0.5 / (c->dzc[k] + c->dzc[k - 1]));
+    h[(nz + 2) + k] = (T)std::min(zc, lz - zc);
+    h[2 * (nz + 2) + k] = (T)(zc < 0.5 * lz ? 1.0 : -1.0);
+  }
+  PLL_HIP_TRY(hipMalloc(&c->d_feat, h.size() * sizeof(T)));
+  PLL_HIP_TRY(hipMemcpy(c->d_feat, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PLL_OK;
+}
+int feat_setup_any(pll_context* c) {
+  return c->prec == PLL_FP64 ? feat_setup<double>(c) : feat_setup<float>(c);
+}
+
+template <typename T>
+FeatMesh<T> feat_mesh(const pll_context* c) {
+  const T* d = static_cast<const T*>(c->d_feat);
+  const int n = c->g.nz + 2;
+  return FeatMesh<T>{d, d + n, d + 2 * n};
+}
+
+template <typename T>
+MeshT<T> mesh_t(const pll_context* c) {
+  const T* d = static_cast<const T*>(c->d_mesh);
+  const int n = c->g.nz + 2;
+  return MeshT<T>{d, d + n, d + 2 * n, d + 3 * n};
+}
+
+template <typename T>
+T* fld(const pll_context* c, int f) {
+  return static_cast<T*>(c->fld[f]);
+}
+
+template <typename T>
+int rk_closure_t(pll_context* c, int stage) {
+  pll_closure* cl = c->closure;
+  PLL_HIP_TRY(hipEventRecord(cl->ready, cl->stream));
+  PLL_HIP_TRY(hipStreamWaitEvent(c->stream, cl->ready, 0));
+  const Geom g = c->g;
+  const int ntx = (g.nx + kLx - 1) / kLx, nty = (g.ny + kLy - 1) / kLy, ntz = (g.nz + kLz - 1) / kLz;
+  const dim3 grid((unsigned)ntx * nty * ntz), block(64 * kWaves);
+  const T dxi = (T)c->dx_inv, dyi = (T)c->dy_inv;
+  const T qdxi = (T)(0.25 * c->dx_inv), qdyi = (T)(0.25 * c->dy_inv);
+  const T rei = (T)c->re_inv, dt = (T)c->dt, ucrf = (T)c->u_crf;
+  const T gs = (T)cl->gs, ys = (T)cl->ys;
+  const float nus = (float)cl->nus;
+  TimerScope ts(c, K_RK_CLOSURE);
+  if (stage == 1)
+    hipLaunchKernelGGL((k_rk_les<T, 1>), grid, block, 0, c->stream, g, fld<T>(c, PLL_U),
+                       fld<T>(c, PLL_V), fld<T>(c, PLL_W), fld<T>(c, PLL_U1), fld<T>(c, PLL_V1),
+                       fld<T>(c, PLL_W1), mesh_t<T>(c), feat_mesh<T>(c), dxi, dyi, qdxi, qdyi, rei,
+                       dt, ucrf, weights(cl), gs, ys, nus, c->d_nut, ntx, nty);
+  else
+    hipLaunchKernelGGL((k_rk_les<T, 2>), grid, block, 0, c->stream, g, fld<T>(c, PLL_U1),
+                       fld<T>(c, PLL_V1), fld<T>(c, PLL_W1), fld<T>(c, PLL_U), fld<T>(c, PLL_V),
+                       fld<T>(c, PLL_W), mesh_t<T>(c), feat_mesh<T>(c), dxi, dyi, qdxi, qdyi, rei,
+                       dt, ucrf, weights(cl), gs, ys, nus, c->d_nut, ntx, nty);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+// The fine and coarse grids nest: integer factors, the same box, and every
+// coarse z cell the union of fz fine ones.
+int check_nesting(const pll_context* f, const pll_context* c, int* fx, int* fy, int* fz) {
+  if (!f || !c) return fail(PLL_ERR_ARG, "null context");
+  const pll_config &a = f->cfg, &b = c->cfg;
+  if (a.nx % b.nx || a.ny % b.ny || a.nz % b.nz)
+    return fail(PLL_ERR_ARG, "fine grid sizes must be integer multiples of the coarse ones");
+  auto same = [](double x, double y) { return std::fabs(x - y) <= 1e-12 * std::fabs(y); };
+  if (!same(a.lx, b.lx) || !same(a.ly, b.ly) || !same(a.lz, b.lz))
+    return fail(PLL_ERR_ARG, "fine and coarse boxes differ");
+  *fx = a.nx / b.nx;
+  *fy = a.ny / b.ny;
+  *fz = a.nz / b.nz;
+  for (int K = 1; K <= b.nz; ++K) {
+    double s = 0.0;
+    for (int q = 1; q <= *fz; ++q) s += f->dzf[(K - 1) * *fz + q];
+    if (!(std::fabs(s - c->dzf[K]) <= 1e-10 * c->dzf[K]))
+      return fail(PLL_ERR_ARG, "coarse z cells are not unions of fine ones");
+  }
+  return PLL_OK;
+}
+
+template <typename Tf, typename Tc>
+int filter_t(pll_context* f, pll_context* c, int fx, int fy, int fz) {
+  const long n = (long)c->g.nx * c->g.ny * c->g.nz;
+  hipLaunchKernelGGL((k_filter<Tf, Tc>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     f->g, fld<Tf>(f, PLL_U), fld<Tf>(f, PLL_V), fld<Tf>(f, PLL_W), f->d_mesh64,
+                     c->g, fld<Tc>(c, PLL_U), fld<Tc>(c, PLL_V), fld<Tc>(c, PLL_W), fx, fy, fz);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+template <typename Tf, typename Tc>
+int samples_t(pll_context* f, pll_context* c, const pll_closure* cl, int fx, int fy, int fz,
+              float* x, float* y) {
+  const long n = (long)c->g.nx * c->g.ny * c->g.nz;
+  const double eps = kLabelEps / (cl->gs * cl->gs);
+  hipLaunchKernelGGL((k_samples<Tf, Tc>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     c->stream, f->g, fld<Tf>(f, PLL_U), fld<Tf>(f, PLL_V), fld<Tf>(f, PLL_W),
+                     f->d_mesh64, c->g, fld<Tc>(c, PLL_U), fld<Tc>(c, PLL_V), fld<Tc>(c, PLL_W),
+                     mesh_t<Tc>(c).dzf_inv, feat_mesh<Tc>(c), (Tc)c->dx_inv, (Tc)c->dy_inv,
+                     (Tc)(0.25 * c->dx_inv), (Tc)(0.25 * c->dy_inv), (Tc)cl->gs, (Tc)cl->ys,
+                     cl->nus, eps, kLabelMax, fx, fy, fz, x, y);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+// filter of the fine context's u, v, w into the coarse one, with its halos
+int filter(pll_context* f, pll_context* c) {
+  int fx, fy, fz;
+  int rc = check_nesting(f, c, &fx, &fy, &fz);
+  if (rc) return rc;
+  PLL_HIP_TRY(hipStreamSynchronize(f->stream));
+  if (f->prec == PLL_FP64)
+    rc = c->prec == PLL_FP64 ? filter_t<double, double>(f, c, fx, fy, fz)
+                             : filter_t<double, float>(f, c, fx, fy, fz);
+  else
+    rc = c->prec == PLL_FP64 ? filter_t<float, double>(f, c, fx, fy, fz)
+                             : filter_t<float, float>(f, c, fx, fy, fz);
+  if (rc) return rc;
+  return launch_bc_vel(c, 0);
+}
+
 }  // namespace
 
+namespace pll {
+int launch_rk_closure(pll_context* c, int stage) {
+  int rc = feat_setup_any(c);
+  if (rc) return rc;
+  if (!c->d_nut) PLL_HIP_TRY(hipMalloc(&c->d_nut, (size_t)c->nspec * sizeof(float)));
+  return c->prec == PLL_FP64 ? rk_closure_t<double>(c, stage) : rk_closure_t<float>(c, stage);
+}
+}  // namespace pll
+
 extern "C" {
+
+int pll_closure_set_scales(pll_closure* c, double gscale, double yscale, double nuscale) {
+  if (!c) return fail(PLL_ERR_ARG, "null closure");
+  if (!(gscale > 0) || !(yscale > 0) || !(nuscale > 0))
+    return fail(PLL_ERR_ARG, "closure scales must be positive");
+  c->gs = gscale;
+  c->ys = yscale;
+  c->nus = nuscale;
+  return PLL_OK;
+}
+
+int pll_closure_get_scales(pll_closure* c, double* gscale, double* yscale, double* nuscale) {
+  if (!c) return fail(PLL_ERR_ARG, "null closure");
+  if (gscale) *gscale = c->gs;
+  if (yscale) *yscale = c->ys;
+  if (nuscale) *nuscale = c->nus;
+  return PLL_OK;
+}
+
+int pll_closure_attach(pll_context* ctx, pll_closure* c) {
+  if (!ctx) return fail(PLL_ERR_ARG, "null context");
+  if (c) {
+    int rc = feat_setup_any(ctx);
+    if (rc) return rc;
+    if (!ctx->d_nut) PLL_HIP_TRY(hipMalloc(&ctx->d_nut, (size_t)ctx->nspec * sizeof(float)));
+    PLL_HIP_TRY(hipMemsetAsync(ctx->d_nut, 0, (size_t)ctx->nspec * sizeof(float), ctx->stream));
+  }
+  ctx->closure = c;
+  return PLL_OK;
+}
+
+int pll_closure_nut(pll_context* ctx, double* host) {
+  if (!ctx || !host) return fail(PLL_ERR_ARG, "null argument");
+  if (!ctx->d_nut) return fail(PLL_ERR_STATE, "no closure attached to this context");
+  std::vector<float> h((size_t)ctx->nspec);
+  PLL_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  PLL_HIP_TRY(hipMemcpy(h.data(), ctx->d_nut, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < h.size(); ++q) host[q] = (double)h[q];
+  return PLL_OK;
+}
+
+int pll_closure_filter(pll_context* fine, pll_context* coarse) {
+  int rc = filter(fine, coarse);
+  if (rc) return rc;
+  PLL_HIP_TRY(hipStreamSynchronize(coarse->stream));
+  return PLL_OK;
+}
+
+int pll_closure_samples(pll_context* fine, pll_context* coarse, pll_closure* cl, float* x,
+                        float* y) {
+  if (!cl || !x || !y) return fail(PLL_ERR_ARG, "null argument");
+  int rc = filter(fine, coarse);
+  if (rc) return rc;
+  if ((rc = feat_setup_any(coarse))) return rc;
+  int fx, fy, fz;
+  check_nesting(fine, coarse, &fx, &fy, &fz);
+  if (fine->prec == PLL_FP64)
+    rc = coarse->prec == PLL_FP64 ? samples_t<double, double>(fine, coarse, cl, fx, fy, fz, x, y)
+                                  : samples_t<double, float>(fine, coarse, cl, fx, fy, fz, x, y);
+  else
+    rc = coarse->prec == PLL_FP64 ? samples_t<float, double>(fine, coarse, cl, fx, fy, fz, x, y)
+                                  : samples_t<float, float>(fine, coarse, cl, fx, fy, fz, x, y);
+  if (rc) return rc;
+  PLL_HIP_TRY(hipStreamSynchronize(coarse->stream));
+  return PLL_OK;
+}
 
 int pll_closure_create(uint64_t seed, pll_closure** out) {
   if (!out) return fail(PLL_ERR_ARG, "null output pointer");
@@ -400,6 +939,8 @@ int pll_closure_create(uint64_t seed, pll_closure** out) {
   };
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) return bail(hip_fail(e, "closure stream"));
+  e = hipEventCreateWithFlags(&c->ready, hipEventDisableTiming);
+  if (e != hipSuccess) return bail(hip_fail(e, "closure event"));
   const size_t pb = (size_t)kNParam * sizeof(float);
   if (hipMalloc(&c->p, pb) != hipSuccess || hipMalloc(&c->m1, pb) != hipSuccess ||
       hipMalloc(&c->m2, pb) != hipSuccess || hipMalloc(&c->grad, pb + sizeof(float)) != hipSuccess ||
@@ -436,6 +977,7 @@ int pll_closure_destroy(pll_closure* c) {
   void* bufs[] = {c->p, c->m1, c->m2, c->wb, c->partial, c->grad};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
+  if (c->ready) (void)hipEventDestroy(c->ready);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PLL_OK;
@@ -493,7 +1035,7 @@ int pll_closure_predict(pll_closure* c, const float* x, long n, float* out) {
   const long nw = waves_for(n);
   hipLaunchKernelGGL((k_mlp<false>), dim3((unsigned)(nw / kWaves)), dim3(64 * kWaves), 0, c->stream,
                      weights(c), x, static_cast<const float*>(nullptr), n, out,
-                     static_cast<float*>(nullptr));
+                     static_cast<float*>(nullptr), tiles_per_wave(n, nw));
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }

@@ -15,7 +15,7 @@ const char* const kKernelNames[K_COUNT] = {
     "rk1",         "rk2",       "calc_rhs",   "fft_x_fwd",    "fft_y_fwd", "solve_trid",
     "fft_y_bwd",   "fft_x_bwd", "correct_vel", "mean_partial", "mean_final", "force_add",
     "bc_vel",      "bc_p",      "check",      "init",         "misc",        "rk2_rhs",
-    "fft_x_fwd_rhs"};
+    "fft_x_fwd_rhs", "rk_closure"};
 
 namespace {
 thread_local std::string g_last_error;
@@ -147,7 +147,7 @@ void release(pll_context* c) {
     if (p) (void)hipFree(p);
   void* bufs[] = {c->spec,   c->d_mesh, c->d_mesh64, c->d_a,    c->d_c,       c->d_piv,
                   c->d_tw_x, c->d_tw_y,    c->d_rtw_x, c->d_twl_x, c->d_twl_y, c->d_rkcoef,
-                  c->d_partial, c->d_scalar, c->d_red};
+                  c->d_partial, c->d_scalar, c->d_red, c->d_feat, c->d_nut};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -300,9 +300,24 @@ int poisson_impl(pll_context* c) {
 int finish_step(pll_context* c, bool fused, bool last);
 
 int step_impl(pll_context* c, bool first, bool last) {
+  int rc;
+  if (c->closure) {
+    // the closure stages (eddy viscosity evaluated in the stage kernel), then
+    // the reference sequence src/main.cc:121-153 unfused
+    if ((rc = launch_rk(c, 1, 0))) return rc;  // :117
+    if ((rc = launch_bc_vel(c, 1))) return rc;  // :121
+    if ((rc = launch_rk(c, 2, 0))) return rc;  // :125
+    if ((rc = launch_bc_vel(c, 0))) return rc;  // :129
+    if ((rc = launch_calc_rhs(c))) return rc;   // :133
+    if ((rc = launch_fft_x_fwd(c))) return rc;  // :137
+    if ((rc = launch_fft_y_fwd(c))) return rc;
+    if ((rc = launch_trid(c))) return rc;
+    if ((rc = launch_fft_y_bwd(c))) return rc;
+    if ((rc = launch_fft_x_bwd_step(c))) return rc;
+    return finish_step(c, false, last);
+  }
   const bool fused = !c->is_forced[1] && !c->is_forced[2];
   const int fold = (fused && !first) ? 1 : 0;
-  int rc;
   if (fused && c->prec != PLL_FP64 && fft_x_rhs_available(c)) {
     // FP32-mixed: the projection of step n (:145), its forcing (:149) and
     // boundary update (:153) are applied by step n+1's stage loads (corr);

@@ -65,6 +65,7 @@ enum KernelId {
   K_MISC,
   K_RK2_RHS,        // stage 2 fused with calcRHS
   K_FFT_X_FWD_RHS,  // calcRHS fused into the forward x transform
+  K_RK_CLOSURE,     // RK stage with the learned eddy viscosity
   K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
@@ -130,6 +131,12 @@ struct pll_context {
   // nz*ny*nx, x fastest); null when not selected
   double* d_stat[11] = {};
 
+  // learned closure (not owned): when attached, the RK stages add the
+  // eddy-viscosity stress with the network evaluated in the stage kernel
+  pll_closure* closure = nullptr;
+  void* d_feat = nullptr;   // feature coefficients per plane (field precision), 3*(nz+2)
+  float* d_nut = nullptr;   // eddy viscosity of the last closure stage, nx*ny*nz
+
   // timers
   bool timing = false;
   std::vector<pll::PendingTimer> pending;
@@ -178,6 +185,8 @@ struct TimerScope {
 // bcload (stage 2, FP32): rebuild the source's halos on load (u1's boundary
 // update not run)
 int launch_rk(pll_context* c, int stage, int fold, int corr = 0, int acc = 0, int bcload = 0);
+// the stage of a context with an attached closure (k_closure.hip)
+int launch_rk_closure(pll_context* c, int stage);
 int rk_setup(pll_context* c);  // per-plane coefficients of the FP32-mixed stencil
 int launch_calc_rhs(pll_context* c);
 // RK stage 2 fused with calcRHS (step sequence only): writes the new u, v, w to
