@@ -395,6 +395,13 @@ __device__ __forceinline__ T& real_at(cpx<T>* buf, int e) {
   return reinterpret_cast<T*>(buf)[2 * pad(e >> 1) + (e & 1)];
 }
 
+// the largest divisor of n that is at most m
+constexpr int chunk_of(int n, int m) {
+  int c = m < n ? m : n;
+  while (n % c) --c;
+  return c;
+}
+
 // The velocity divergence of calcRHS (src/mod_calcRHS.cc:7-33), for a forward
 // x transform that reads it straight from u, v, w instead of from p.
 template <typename T>
@@ -437,25 +444,46 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xf(Geom g, cons
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < NB * N; e += NT) {
-    const int s = e / N, i = e % N;
-    const long b = rowbase[s];
-    T val = T(0);
-    if (b >= 0) {
-      const long c = b + i;
-      if (RHS) {
-        // src/mod_calcRHS.cc:21-28, same association as k_calc_rhs
-        const long cw = (i == 0 && rs.bcfold) ? b + (N - 1) : c - 1;  // u(0) = u(nx)
-        const unsigned char wf = roww[s];
-        const T wt = (wf & 2) ? T(0) : rs.w[c];
-        const T wb = (wf & 1) ? T(0) : rs.w[c - g.sxy];
-        val = (rs.u[c] - rs.u[cw]) * rs.dtidxi + (rs.v[c] - rs.v[c + rowvs[s]]) * rs.dtidyi +
-              (wt - wb) * rowdz[s];
-      } else {
-        val = p[c];
+  // NB*N values, NLD per thread, loaded in chunks of CH whose loads are all
+  // issued before the first is used
+  constexpr int NLD = NB * N / NT, CH = RHS ? chunk_of(NLD, 6) : NLD;
+  static_assert((NB * N) % NT == 0 && NLD % CH == 0, "whole rows per block");
+#pragma unroll
+  for (int u0 = 0; u0 < NLD; u0 += CH) {
+    T a0[CH], a1[CH], a2[CH], a3[CH], a4[CH], a5[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int e = threadIdx.x + (u0 + u) * NT;
+      const int s = e / N, i = e % N;
+      const long b = rowbase[s];
+      a0[u] = a1[u] = a2[u] = a3[u] = a4[u] = a5[u] = T(0);
+      if (b >= 0) {
+        const long c = b + i;
+        if (RHS) {
+          const long cw = (i == 0 && rs.bcfold) ? b + (N - 1) : c - 1;  // u(0) = u(nx)
+          const unsigned char wf = roww[s];
+          a0[u] = rs.u[c];
+          a1[u] = rs.u[cw];
+          a2[u] = rs.v[c];
+          a3[u] = rs.v[c + rowvs[s]];
+          if (!(wf & 2)) a4[u] = rs.w[c];
+          if (!(wf & 1)) a5[u] = rs.w[c - g.sxy];
+        } else {
+          a0[u] = p[c];
+        }
       }
     }
-    real_at(buf, s * L * 2 + i) = val;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int e = threadIdx.x + (u0 + u) * NT;
+      const int s = e / N, i = e % N;
+      T val = a0[u];
+      if (RHS) {
+        // src/mod_calcRHS.cc:21-28, same association as k_calc_rhs
+        val = (a0[u] - a1[u]) * rs.dtidxi + (a2[u] - a3[u]) * rs.dtidyi + (a4[u] - a5[u]) * rowdz[s];
+      }
+      real_at(buf, s * L * 2 + i) = val;
+    }
   }
   __syncthreads();
   const int s = threadIdx.x / TPS, jj = threadIdx.x % TPS;
@@ -498,10 +526,18 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xb(Geom g, cons
     rowbase[threadIdx.x] = r < nrows ? g.idx(HALO ? 0 : 1, j + 1, k + 1) : -1;
   }
   const cpx<T>* specc = reinterpret_cast<const cpx<T>*>(spec);
-  for (int e = threadIdx.x; e < NB * L; e += NT) {
-    const int s = e / L, kk = e % L;
-    const long r = row0 + s;
-    buf[pad(s * L + kk)] = r < nrows ? specc[r * L + kk] : cpx<T>{T(0), T(0)};
+  {
+    constexpr int NLD = NB * L / NT;
+    static_assert((NB * L) % NT == 0, "whole rows per block");
+    cpx<T> t[NLD];
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) {
+      const int e = threadIdx.x + u * NT;
+      const long r = row0 + e / L;
+      t[u] = r < nrows ? specc[r * L + e % L] : cpx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) buf[pad(threadIdx.x + u * NT)] = t[u];
   }
   __syncthreads();
   const int s = threadIdx.x / TPS, jj = threadIdx.x % TPS;
@@ -532,21 +568,71 @@ __global__ __launch_bounds__((x_rows<T, L>() * tps<L>())) void k_xb(Geom g, cons
   stockham<T, +1, L, 1>(buf, twL, s * L, jj);
   if (HALO) {
     // pair q = (p(2q), p(2q+1)); p(i) is element i-1 of the row, p(0) = p(nx), p(nx+1) = p(1)
-    constexpr int NP = L + 1;
-    for (int e = threadIdx.x; e < NB * NP; e += NT) {
+    constexpr int NP = L + 1, NST = (NB * NP + NT - 1) / NT;
+    cpx<T> t[NST];
+#pragma unroll
+    for (int u = 0; u < NST; ++u) {
+      const int e = threadIdx.x + u * NT;
       const int s2 = e / NP, q = e % NP;
-      const long b = rowbase[s2];
-      if (b < 0) continue;
-      const int e0 = q == 0 ? N - 1 : 2 * q - 1, e1 = q == L ? 0 : 2 * q;
-      cpx<T> v{norm * real_at(buf, s2 * N + e0), norm * real_at(buf, s2 * N + e1)};
-      reinterpret_cast<cpx<T>*>(p + b)[q] = v;
+      if (e < NB * NP) {
+        const int e0 = q == 0 ? N - 1 : 2 * q - 1, e1 = q == L ? 0 : 2 * q;
+        t[u] = cpx<T>{norm * real_at(buf, s2 * N + e0), norm * real_at(buf, s2 * N + e1)};
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NST; ++u) {
+      const int e = threadIdx.x + u * NT;
+      const int s2 = e / NP, q = e % NP;
+      if (e < NB * NP && rowbase[s2] >= 0) reinterpret_cast<cpx<T>*>(p + rowbase[s2])[q] = t[u];
     }
   } else {
-    for (int e = threadIdx.x; e < NB * N; e += NT) {
-      const int s2 = e / N, i = e % N;
-      const long b = rowbase[s2];
-      if (b >= 0) p[b + i] = norm * real_at(buf, s2 * N + i);
+    constexpr int NST = NB * N / NT;
+    static_assert((NB * N) % NT == 0, "whole rows per block");
+    T t[NST];
+#pragma unroll
+    for (int u = 0; u < NST; ++u) t[u] = norm * real_at(buf, threadIdx.x + u * NT);
+#pragma unroll
+    for (int u = 0; u < NST; ++u) {
+      const int e = threadIdx.x + u * NT;
+      const long b = rowbase[e / N];
+      if (b >= 0) p[b + e % N] = t[u];
     }
+  }
+}
+
+// A y tile between global memory and LDS: NE complex slots, slot e = j*Q + q2
+// is the pair of columns (2 q2, 2 q2 + 1) of row j.  Each thread first issues
+// all of its loads (or LDS reads), then all of its LDS writes (or stores), so
+// a thread keeps ceil(NE/NT) memory operations in flight instead of one.
+template <typename T, int NE, int NT, int Q>
+__device__ __forceinline__ void load_tile(cpx<T>* buf, const T* plane, int nx) {
+  constexpr int NLD = (NE + NT - 1) / NT;
+  cpx<T> t[NLD];
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int e = threadIdx.x + u * NT;
+    if (NE % NT == 0 || e < NE)
+      t[u] = reinterpret_cast<const cpx<T>*>(plane + (long)(e / Q) * nx)[e % Q];
+  }
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int e = threadIdx.x + u * NT;
+    if (NE % NT == 0 || e < NE) buf[pad(e)] = t[u];
+  }
+}
+template <typename T, int NE, int NT, int Q>
+__device__ __forceinline__ void store_tile(const cpx<T>* buf, T* plane, int nx) {
+  constexpr int NLD = (NE + NT - 1) / NT;
+  cpx<T> t[NLD];
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int e = threadIdx.x + u * NT;
+    if (NE % NT == 0 || e < NE) t[u] = buf[pad(e)];
+  }
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int e = threadIdx.x + u * NT;
+    if (NE % NT == 0 || e < NE) reinterpret_cast<cpx<T>*>(plane + (long)(e / Q) * nx)[e % Q] = t[u];
   }
 }
 
@@ -561,14 +647,14 @@ __global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yf(Geom g, T* _
   const int tile = (int)(b % (unsigned)ntiles), k = (int)(b / (unsigned)ntiles);
   T* plane = spec + (long)k * g.nx * g.ny + (long)tile * W;
   // a slot holds the values of columns (2q, 2q+1) of one row: one 2-element load
-  for (int e = threadIdx.x; e < L * Q; e += NT) {
-    const int j = e / Q, q2 = e % Q;
-    buf[pad(j * Q + q2)] = reinterpret_cast<const cpx<T>*>(plane + (long)j * g.nx)[q2];
-  }
+  load_tile<T, L * Q, NT, Q>(buf, plane, g.nx);
   __syncthreads();
   const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
   stockham<T, -1, L, Q>(buf, twL, q, jj);
-  for (int m = jj; m <= L / 2; m += TPS) {
+#pragma unroll
+  for (int mu = 0; mu <= L / 2 / TPS; ++mu) {
+    const int m = jj + mu * TPS;
+    if (m > L / 2) break;
     const cpx<T> zm = buf[pad(m * Q + q)];
     const cpx<T> zc = buf[pad(((L - m) % L) * Q + q)];
     const T ar = T(0.5) * (zm.re + zc.re), ai = T(0.5) * (zm.im - zc.im);
@@ -596,10 +682,7 @@ __global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yb(Geom g, T* _
   const int tile = (int)(b % (unsigned)ntiles), k = (int)(b / (unsigned)ntiles);
   T* plane = spec + (long)k * g.nx * g.ny + (long)tile * W;
   // a slot holds the values of columns (2q, 2q+1) of one row: one 2-element load
-  for (int e = threadIdx.x; e < L * Q; e += NT) {
-    const int j = e / Q, q2 = e % Q;
-    buf[pad(j * Q + q2)] = reinterpret_cast<const cpx<T>*>(plane + (long)j * g.nx)[q2];
-  }
+  load_tile<T, L * Q, NT, Q>(buf, plane, g.nx);
   __syncthreads();
   const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
   // slot (j, q) holds (perm_a[j], perm_b[j]) of columns 2q and 2q+1
@@ -624,10 +707,7 @@ __global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yb(Geom g, T* _
   for (int u = 0; u < RMIN; ++u) buf[pad((jj + u * TPS) * Q + q)] = z[u];
   __syncthreads();
   stockham<T, +1, L, Q>(buf, twL, q, jj);
-  for (int e = threadIdx.x; e < L * Q; e += NT) {
-    const int j = e / Q, q2 = e % Q;
-    reinterpret_cast<cpx<T>*>(plane + (long)j * g.nx)[q2] = buf[pad(j * Q + q2)];
-  }
+  store_tile<T, L * Q, NT, Q>(buf, plane, g.nx);
 }
 
 // ---- host side -------------------------------------------------------------------
