@@ -676,36 +676,45 @@ __global__ __launch_bounds__((y_seqs<T, L>() * tps<L>())) void k_yb(Geom g, T* _
                                                                   const cpx<T>* __restrict__ twL,
                                                                   int ntiles) {
   constexpr int TPS = tps<L>(), Q = y_seqs<T, L>(), NT = Q * TPS, W = 2 * Q;
-  constexpr int RMIN = shape_rmin<Shape<L>>();
   __shared__ cpx<T> buf[padded(L * Q)];
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = (int)(b % (unsigned)ntiles), k = (int)(b / (unsigned)ntiles);
   T* plane = spec + (long)k * g.nx * g.ny + (long)tile * W;
-  // a slot holds the values of columns (2q, 2q+1) of one row: one 2-element load
-  load_tile<T, L * Q, NT, Q>(buf, plane, g.nx);
-  __syncthreads();
-  const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
-  // slot (j, q) holds (perm_a[j], perm_b[j]) of columns 2q and 2q+1
-  cpx<T> z[RMIN];
+  // Global row j, column pair q holds (perm_a[j], perm_b[j]) of columns 2q and
+  // 2q+1.  Rows 2m and 2m+1 (m = 1..L/2-1) give the complex inputs Z[m] and
+  // Z[L-m] of the pair's transform: with A = (e0.re, e1.re), B = (e0.im,
+  // e1.im), Z = A + iB below L/2 and conj(A) + i conj(B) above; rows 0 and 1
+  // hold Z[0] and Z[L/2].  Each thread loads its row pairs and writes the
+  // unpacked values to LDS directly.
+  {
+    constexpr int NE = (L / 2) * Q, NLD = (NE + NT - 1) / NT;
+    cpx<T> e0[NLD], e1[NLD];
 #pragma unroll
-  for (int u = 0; u < RMIN; ++u) {
-    const int m = jj + u * TPS;
-    if (m == 0) {
-      z[u] = buf[pad(q)];
-    } else if (m == L / 2) {
-      z[u] = buf[pad(Q + q)];
-    } else {
-      const int mm = m < L / 2 ? m : L - m;
-      const cpx<T> e0 = buf[pad(2 * mm * Q + q)], e1 = buf[pad((2 * mm + 1) * Q + q)];
-      // A = (e0.re, e1.re), B = (e0.im, e1.im); Z = A + iB, or conj(A) + i conj(B) past L/2
-      z[u] = m < L / 2 ? cpx<T>{e0.re - e1.im, e1.re + e0.im}
-                       : cpx<T>{e0.re + e1.im, e0.im - e1.re};
+    for (int u = 0; u < NLD; ++u) {
+      const int e = threadIdx.x + u * NT;
+      if (NE % NT == 0 || e < NE) {
+        const int mm = e / Q, q2 = e % Q;
+        e0[u] = reinterpret_cast<const cpx<T>*>(plane + (long)(2 * mm) * g.nx)[q2];
+        e1[u] = reinterpret_cast<const cpx<T>*>(plane + (long)(2 * mm + 1) * g.nx)[q2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) {
+      const int e = threadIdx.x + u * NT;
+      if (NE % NT == 0 || e < NE) {
+        const int mm = e / Q, q2 = e % Q;
+        if (mm == 0) {
+          buf[pad(q2)] = e0[u];
+          buf[pad((L / 2) * Q + q2)] = e1[u];
+        } else {
+          buf[pad(mm * Q + q2)] = cpx<T>{e0[u].re - e1[u].im, e1[u].re + e0[u].im};
+          buf[pad((L - mm) * Q + q2)] = cpx<T>{e0[u].re + e1[u].im, e0[u].im - e1[u].re};
+        }
+      }
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int u = 0; u < RMIN; ++u) buf[pad((jj + u * TPS) * Q + q)] = z[u];
-  __syncthreads();
+  const int q = threadIdx.x % Q, jj = threadIdx.x / Q;
   stockham<T, +1, L, Q>(buf, twL, q, jj);
   store_tile<T, L * Q, NT, Q>(buf, plane, g.nx);
 }
