@@ -1,0 +1,64 @@
+"""FP32-mixed against FP64 in fully developed turbulence, statistically.
+
+A turbulent channel at Re_tau~395 (384x256x192, 2pi x pi x 2, tanh stretch
+1.5, Re_b = 13750, dt = 0.004) is spun up from a perturbed laminar start; the
+FP64 and the FP32-mixed paths then continue from that same state for 40 time
+units and sample the calcStat sums every 20 steps.  The trajectories
+decorrelate (turbulence amplifies round-off), so the comparison is of the
+time- and plane-averaged statistics: the mean velocity profile (the 0.5 %
+bar), the Reynolds stresses and the wall shear.  The FP64 GPU path is the
+oracle's arithmetic bit for bit (tests/test_gpu_parity.py), so its statistics
+stand for the oracle's over a horizon the CPU oracle cannot run."""
+import math
+
+import numpy as np
+import pytest
+
+import powerllel_channel_sycl_amd as pll
+from powerllel_channel_sycl_amd import closure_pipeline as cp
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ, LX, LY, LZ, RE, DT = 384, 256, 192, 2 * math.pi, math.pi, 2.0, 6875.0, 0.004
+SPIN, STEPS, EVERY = 7500, 10000, 20
+
+
+@pytest.mark.timeout(300)
+def test_fp32_statistics_match_fp64_in_turbulence():
+    print("CASE fp32 vs fp64 turbulent statistics Re_tau~395", flush=True)
+    dzf = pll.mesh_by_func(1, 0, 1.5, NZ, LZ)
+    zc = np.cumsum(np.concatenate([[0.5 * dzf[1]], 0.5 * (dzf[1:NZ] + dzf[2:NZ + 1])]))
+    with pll.Channel(NX, NY, NZ, LX, LY, LZ, RE, DT, dzf, precision="fp32_mixed") as ch:
+        ch.initFlow("poi", with_noise=False)
+        cp.seed_turbulence(ch, 0.3, 1, LX, LY, zc)
+        for s in range(0, SPIN, 2500):
+            ch.step(2500)
+            print(f"  spin-up {s + 2500}/{SPIN}", flush=True)
+        state = {f: ch.download(f) for f in ("u", "v", "w", "p")}
+    stats = {}
+    for prec in ("fp64", "fp32_mixed"):
+        with pll.Channel(NX, NY, NZ, LX, LY, LZ, RE, DT, dzf, precision=prec) as ch:
+            for f, a in state.items():
+                ch.upload(f, a)
+            ch.stat_enable((1, 1, 1, 1, 1, 1, 0, 1, 0, 0, 0))
+            n = STEPS // EVERY
+            for s in range(n):
+                ch.step(EVERY)
+                ch.stat_accumulate()
+                if (s + 1) % (n // 4) == 0:
+                    print(f"  {prec}: {(s + 1) * EVERY}/{STEPS} steps", flush=True)
+            assert ch.checkNaN("u")
+            st = cp.plane_stats(ch, n)
+            st["re_tau"] = cp.wall_re_tau(st["U"], dzf[1], RE)
+            stats[prec] = st
+    a, b = stats["fp64"], stats["fp32_mixed"]
+    prof = float(np.max(np.abs(b["U"] - a["U"])) / np.max(a["U"]))
+    errs = {k: float(np.max(np.abs(b[k] - a[k])) / np.max(np.abs(a[k]))) for k in ("uu", "vv", "ww", "uw")}
+    dre = abs(b["re_tau"] - a["re_tau"]) / a["re_tau"]
+    print(f"  Re_tau fp64 {a['re_tau']:.1f} fp32 {b['re_tau']:.1f}; mean-profile err {prof:.2e}; "
+          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert 360 < a["re_tau"] < 420  # turbulent at the target friction Reynolds number
+    assert prof < 5e-3              # the 0.5 % mean-profile bar
+    assert dre < 0.02
+    for k, v in errs.items():
+        assert v < 0.05, (k, v)     # Reynolds stresses, sampling noise of 40 time units
