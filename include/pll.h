@@ -198,7 +198,7 @@ int pll_stat_upload(pll_context *ctx, int var, const double *host);
 int pll_stat_plane_sums(pll_context *ctx, int var, double *host);
 
 /* ---- learned closure ------------------------------------------------------------ */
-/* No counterpart in the reference.  A 10 -> 64 -> 64 -> 1 perceptron (tanh
+/* No counterpart in the reference.  A 10 -> 64 -> 64 -> 1 perceptron (ReLU
  * hidden layers, softplus output) with BF16 weights and FP32 accumulation on
  * MFMA, predicting a normalised eddy viscosity from 10 normalised features
  * (9 filtered velocity gradients, wall distance).  Parameters (FP32 master):

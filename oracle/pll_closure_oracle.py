@@ -14,7 +14,7 @@ gradients against analytic fields at second order; the box filter against
 exact divergence preservation; box stresses against numpy's weighted
 covariance; labels against constructed stress-strain pairs.
 
-10 -> 64 -> 64 -> 1, tanh hidden layers, softplus output, loss = mean of
+10 -> 64 -> 64 -> 1, ReLU hidden layers, softplus output, loss = mean of
 (softplus(o) - y)^2.  With bf16=True the inputs, weights, activations h1 and h2
 and the back-propagated deltas are rounded to BF16 (nearest, ties to even) at
 the points where the GPU rounds them before an MFMA product; sums are FP64.
@@ -62,11 +62,11 @@ def forward(p, x, bf16_round=True):
     xb = np.zeros((x.shape[0], N_IN_PAD))
     xb[:, :N_IN] = _r(x, bf16_round)
     W1, W2, W3 = (_r(w[k], bf16_round) for k in ("W1", "W2", "W3"))
-    h1 = np.tanh(xb @ W1 + w["b1"])
+    h1 = np.maximum(xb @ W1 + w["b1"], 0.0)
     if bf16_round:
         h1 = h1.astype(np.float32).astype(np.float64)
     h1b = _r(h1, bf16_round)
-    h2 = np.tanh(h1b @ W2 + w["b2"])
+    h2 = np.maximum(h1b @ W2 + w["b2"], 0.0)
     if bf16_round:
         h2 = h2.astype(np.float32).astype(np.float64)
     h2b = _r(h2, bf16_round)
@@ -94,11 +94,11 @@ def gradient(p, x, y, bf16_round=True):
     grad = np.zeros(N_PARAM)
     grad[O_W3:O_B3] = f["h2b"].T @ g
     grad[O_B3] = g.sum()
-    d2 = g[:, None] * f["W3"][None, :] * (1.0 - f["h2"] ** 2)
+    d2 = np.where(f["h2"] > 0, g[:, None] * f["W3"][None, :], 0.0)
     grad[O_B2:O_W3] = d2.sum(axis=0)
     d2b = _r(d2, bf16_round)
     grad[O_W2:O_B2] = (f["h1b"].T @ d2b).ravel()
-    d1 = (d2b @ f["W2"].T) * (1.0 - f["h1"] ** 2)
+    d1 = np.where(f["h1"] > 0, d2b @ f["W2"].T, 0.0)
     grad[O_B1:O_W2] = d1.sum(axis=0)
     d1b = _r(d1, bf16_round)
     gW1 = f["xb"].T @ d1b

@@ -1,5 +1,5 @@
 // k_closure.hip -- the learned eddy-viscosity closure: a 10 -> 64 -> 64 -> 1
-// multilayer perceptron (tanh hidden layers, softplus output) with BF16
+// multilayer perceptron (ReLU hidden layers, softplus output) with BF16
 // weights and FP32 accumulation on the MFMA units of gfx950, its training
 // (forward, backward, Adam) and its inference.  The reference has no closure.
 //
@@ -35,18 +35,14 @@ constexpr int kH = 64;    // hidden width
 constexpr int oW1 = 0, oB1 = oW1 + kInP * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH,
               oW3 = oB2 + kH, oB3 = oW3 + kH, kNParam = oB3 + 1;
 constexpr int kWaves = 4;          // waves per block
-constexpr long kMaxWaves = 4096;   // training grid (partials are kMaxWaves x kNParam)
+constexpr long kMaxWaves = 1024;   // training grid: one wave per SIMD (partials kMaxWaves x kNParam)
+constexpr int kRedChunk = 32;      // wave partials summed per chunk in the first reduction level
 
 typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-__host__ __device__ inline uint16_t to_bf16(float x) {
-  uint32_t u;
-  memcpy(&u, &x, 4);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
-  u += 0x7fffu + ((u >> 16) & 1u);                                  // nearest, ties to even
-  return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest, ties to even (gfx950 v_cvt_pk_bf16_f32).
+__device__ inline uint16_t to_bf16(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 __host__ __device__ inline float from_bf16(uint16_t b) {
   const uint32_t u = (uint32_t)b << 16;
   float x;
@@ -96,7 +92,7 @@ __device__ inline void stage_x(const float* __restrict__ x, long m0, long n, L_&
 }
 
 // Forward pass of one 32-sample tile whose BF16 features are in L.x (written
-// before the call by any lane of the wave): fills L.h1/h2, returns the tanh
+// before the call by any lane of the wave): fills L.h1/h2, returns the ReLU
 // activations in D layout and, on lanes 0..31, the output o of sample lane.
 // Every wave of the block calls it the same number of times (it holds barriers).
 template <class L_>
@@ -104,7 +100,7 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
                                     float& out_o) {
   __syncthreads();
   const int col = lane % 32, kh = 8 * (lane / 32);
-  // layer 1: h1 = tanh(X W1 + b1)
+  // layer 1: h1 = relu(X W1 + b1)
   const bf16x8 a1 = load8(&L.x[col][kh]);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -116,14 +112,14 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
     const float bias = w.p[oB1 + 32 * t + col];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const float h = tanhf(acc[j] + bias);
+      const float h = fmaxf(acc[j] + bias, 0.f);
       acc[j] = h;
       L.h1[drow(j, lane)][32 * t + col] = to_bf16(h);
     }
     h1[t] = acc;
   }
   __syncthreads();
-  // layer 2: h2 = tanh(h1 W2 + b2)
+  // layer 2: h2 = relu(h1 W2 + b2)
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     f32x16 acc = {};
@@ -138,7 +134,7 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
     const float bias = w.p[oB2 + 32 * t + col];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const float h = tanhf(acc[j] + bias);
+      const float h = fmaxf(acc[j] + bias, 0.f);
       acc[j] = h;
       L.h2[drow(j, lane)][32 * t + col] = to_bf16(h);
     }
@@ -196,7 +192,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
       L.g[lane] = g;
     }
     __syncthreads();
-    // output layer gradients and delta2 = g w3 (1 - h2^2)
+    // output layer gradients and delta2 = g w3 relu'(z2)
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int nn = 32 * t + col;
@@ -207,7 +203,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
         const float g = L.g[m];
         gw3[t] += from_bf16(L.h2[m][nn]) * g;
         const float h = h2[t][j];
-        const float d2 = g * w3n * (1.f - h * h);
+        const float d2 = h > 0.f ? g * w3n : 0.f;
         gb2[t] += d2;
         L.d2[m][nn] = to_bf16(d2);
       }
@@ -230,7 +226,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
           }
           gW2[tk][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, gW2[tk][tn], 0, 0, 0);
         }
-    // delta1 = (d2 W2^T) (1 - h1^2) : rows m, cols k, K = n
+    // delta1 = (d2 W2^T) relu'(z1) : rows m, cols k, K = n
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       f32x16 acc = {};
@@ -243,7 +239,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const float h = h1[t][j];
-        const float d1 = acc[j] * (1.f - h * h);
+        const float d1 = h > 0.f ? acc[j] : 0.f;
         gb1[t] += d1;
         L.d1[drow(j, lane)][32 * t + col] = to_bf16(d1);
       }
@@ -303,13 +299,25 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
   }
 }
 
-// Sum of the per-wave partials in wave order: grad[i] for i < kNParam, loss sum last.
+// Sum of the per-wave partials in a fixed order, in two levels: chunk c of
+// kRedChunk waves into chunks[c], then the chunks in order into grad
+// (grad[i] for i < kNParam, the loss sum last).
 __global__ __launch_bounds__(256) void k_grad_reduce(const float* __restrict__ partial,
-                                                     long nwaves, float* __restrict__ grad) {
+                                                     long nwaves, float* __restrict__ chunks) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const long c = blockIdx.y;
+  if (i > kNParam) return;
+  const long q1 = std::min(nwaves, (c + 1) * kRedChunk);
+  float s = 0.f;
+  for (long q = c * kRedChunk; q < q1; ++q) s += partial[q * (kNParam + 1) + i];
+  chunks[c * (kNParam + 1) + i] = s;
+}
+__global__ __launch_bounds__(256) void k_grad_final(const float* __restrict__ chunks, long nchunk,
+                                                    float* __restrict__ grad) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i > kNParam) return;
   float s = 0.f;
-  for (long q = 0; q < nwaves; ++q) s += partial[q * (kNParam + 1) + i];
+  for (long c = 0; c < nchunk; ++c) s += chunks[c * (kNParam + 1) + i];
   grad[i] = s;
 }
 
@@ -666,7 +674,8 @@ struct pll_closure {
   uint16_t* wb = nullptr;  // BF16 W1 | W2 | W3
   float* partial = nullptr;
   long npartial_waves = 0;
-  float* grad = nullptr;  // kNParam + 1 (loss sum last)
+  float* grad = nullptr;    // kNParam + 1 (loss sum last)
+  float* chunks = nullptr;  // first-level gradient sums, (kMaxWaves / kRedChunk) x (kNParam + 1)
   long step = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ready = nullptr;  // orders a context's closure stages after training
@@ -709,8 +718,11 @@ int gradient(pll_closure* c, const float* x, const float* y, long n) {
   hipLaunchKernelGGL((k_mlp<true>), dim3((unsigned)(nw / kWaves)), dim3(64 * kWaves), 0, c->stream,
                      weights(c), x, y, n, static_cast<float*>(nullptr), c->partial,
                      tiles_per_wave(n, nw));
-  hipLaunchKernelGGL(k_grad_reduce, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream,
-                     c->partial, nw, c->grad);
+  const long nchunk = (nw + kRedChunk - 1) / kRedChunk;
+  hipLaunchKernelGGL(k_grad_reduce, dim3((kNParam + 1 + 255) / 256, (unsigned)nchunk), dim3(256), 0,
+                     c->stream, c->partial, nw, c->chunks);
+  hipLaunchKernelGGL(k_grad_final, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream,
+                     c->chunks, nchunk, c->grad);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
@@ -944,6 +956,7 @@ int pll_closure_create(uint64_t seed, pll_closure** out) {
   const size_t pb = (size_t)kNParam * sizeof(float);
   if (hipMalloc(&c->p, pb) != hipSuccess || hipMalloc(&c->m1, pb) != hipSuccess ||
       hipMalloc(&c->m2, pb) != hipSuccess || hipMalloc(&c->grad, pb + sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->chunks, (size_t)(kMaxWaves / kRedChunk) * (pb + sizeof(float))) != hipSuccess ||
       hipMalloc(&c->wb, (size_t)(kInP * kH + kH * kH + kH) * sizeof(uint16_t)) != hipSuccess)
     return bail(fail(PLL_ERR_NOMEM, "closure parameters"));
   // Glorot-uniform weights, zero biases (the output bias starts at softplus^-1(1))
@@ -974,7 +987,7 @@ int pll_closure_create(uint64_t seed, pll_closure** out) {
 int pll_closure_destroy(pll_closure* c) {
   if (!c) return PLL_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->p, c->m1, c->m2, c->wb, c->partial, c->grad};
+  void* bufs[] = {c->p, c->m1, c->m2, c->wb, c->partial, c->grad, c->chunks};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (c->ready) (void)hipEventDestroy(c->ready);

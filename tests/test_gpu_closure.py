@@ -25,9 +25,10 @@ def _dev(a):
 
 
 def _close_to_oracle(got, want):
-    """GPU tanhf and the oracle's tanh may differ by an FP32 ulp; where that lands
-    on a BF16 rounding boundary of h1 or h2, one activation moves by a BF16 step.
-    So: nearly all outputs to 1e-4, the rare rest within a few BF16 steps."""
+    """The GPU sums the hidden pre-activations in FP32 on MFMA, the oracle in
+    FP64; where the difference lands on a BF16 rounding boundary of h1 or h2,
+    one activation moves by a BF16 step.  So: nearly all outputs to 1e-4, the
+    rare rest within a few BF16 steps."""
     err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3)
     assert np.mean(err > 1e-4) < 0.01, np.mean(err > 1e-4)
     assert np.max(err) < 1e-2, np.max(err)
