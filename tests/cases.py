@@ -21,6 +21,8 @@ CASES = {
     "c2": (256, 192, 128, 4 * math.pi, 2 * math.pi, 2.0, 1.5, 2800.0, 1e-3),
     "c3slab": (768, 384, 8, 2 * math.pi, math.pi, 2.0, 1.8, 10000.0, 5e-4),
     "c4slab": (1536, 768, 4, 2 * math.pi, math.pi, 2.0, 2.0, 20000.0, 2.5e-4),
+    # a tall column (nz = 300)
+    "tall": (16, 12, 300, 2 * math.pi, math.pi, 2.0, 1.8, 2800.0, 2e-3),
 }
 
 

@@ -58,6 +58,25 @@ def test_fp32_stages_close_to_fp64(name):
         assert rel_err(a - a.mean(), b - b.mean()) < 1e-4
 
 
+@pytest.mark.parametrize("name", ["c1", "c2", "tall"])
+def test_fp32_trid_matches_oracle(name):
+    """The FP32-mixed tridiagonal solve (FP64 arithmetic with reciprocal
+    pivots, FP32 work array) against the FP64 oracle on the same random
+    right-hand side, for short and tall columns."""
+    print("CASE fp32 trid", name, flush=True)
+    o = make_oracle(name)
+    with make_channel(name, precision="fp32_mixed") as ch:
+        rng = np.random.default_rng(5)
+        spec = rng.standard_normal((ch.nz, ch.ny, ch.nx)).astype(np.float32).astype(np.float64)
+        ch.upload_spec(spec)
+        ch.solve_trid()
+        got = ch.download_spec()
+    o.solve_trid(spec)
+    err = np.max(np.abs(got - spec)) / np.max(np.abs(spec))
+    print(f"  max error / max |x| = {err:.2e}")
+    assert err < 1e-5
+
+
 @pytest.mark.parametrize("name,steps", [("c1", 10), ("r3", 10)])
 def test_fp32_short_horizon_pointwise(name, steps):
     """10 steps: u, v, w within 1e-5 (relative to max|u|) of the FP64 oracle,
