@@ -199,8 +199,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU sharding is a design, see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
+                                   "(sharding across GPUs is designed, not built)"}))
         sys.exit(2)
     main64 = measure(args.workload, "fp64", args.steps, args.warmup)
     result = {

@@ -144,8 +144,7 @@ def load_library(path=LIB_PATH):
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
-        raise ImportError(f"{path} is not built: run __graft_entry__.build() "
-                          "(make -C powerllel_channel_sycl_amd/csrc)")
+        raise ImportError(f"{path} is not built: run make -C powerllel_channel_sycl_amd/csrc")
     lib = C.CDLL(path)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -7,7 +7,7 @@
    (``pll_closure_samples``): the filtered field's features and the
    least-squares eddy-viscosity labels of the box subgrid stress go to a
    device dataset, and the filtered field itself is sampled into the coarse
-   observer's statistics (the judge: filtered-DNS mean profile and stresses).
+   observer's statistics (the reference: filtered-DNS mean profile and stresses).
 3. The network is trained on the GPU by the library's own MFMA forward and
    backward kernels with Adam, one snapshot per step; the last snapshots are
    held out.

@@ -1,9 +1,9 @@
 """Read and write the checkpoint files (one FP64 (nz, ny, nx) dataset plus
 small int datasets) with the serial HDF5 C library through ctypes.
 
-Test infrastructure: an independent path to the files the host driver writes
+Test infrastructure: an independent path to the files pll_channel writes
 (src/mod_hdf5.cc:87-338 layout), so that a wrong shape, order or timestamp in
-the driver's writer is not hidden by the same mistake in its reader.
+its writer is not hidden by the same mistake in its reader.
 """
 import ctypes as C
 import os

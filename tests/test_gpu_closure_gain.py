@@ -4,7 +4,7 @@ powerllel_channel_sycl_amd/closure_pipeline.py at Re_tau~395): a fine DNS
 snapshots (factors 4x4x3) feeding the training set and the filtered-DNS
 statistics, Adam training of the perceptron on the GPU, then two coarse
 96x64x64 runs from the filtered DNS state -- without a model and with the
-closure in the stage kernels -- judged against the filtered DNS mean profile,
+closure in the stage kernels -- compared with the filtered DNS mean profile,
 Reynolds stresses and wall shear.  The runs are deterministic (fixed seeds,
 fixed-order reductions), so the comparison is reproducible on one GPU."""
 import json

@@ -100,8 +100,8 @@ def test_fp32_short_horizon_pointwise(name, steps):
 
 
 def test_fp32_mean_profile_and_stress_long_horizon():
-    """200 steps of the 64^3 channel: mean velocity profile within 0.5% (north-star
-    bar) and the u'w' stress profile within 1% of its peak, against the FP64 oracle."""
+    """200 steps of the 64^3 channel: mean velocity profile within 0.5% and the
+    u'w' stress profile within 1% of its peak, against the FP64 oracle."""
     name, steps = "c1", 200
     print("CASE fp32 long", name, steps, flush=True)
     o, f = _seeded(name)

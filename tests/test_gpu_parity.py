@@ -4,7 +4,7 @@ Each routine gets the same host inputs on both sides.  Stencils, halos and the
 tridiagonal solve keep the reference's operation order and are compared bit
 for bit; the FFT stages use a different (but exact-in-arithmetic) factorisation
 than the oracle and are compared to 1e-13 relative; the full RK2 step is
-compared after 10 steps at the north-star bar of 1e-11 relative.
+compared after 10 steps at 1e-11 relative.
 """
 import numpy as np
 import pytest
