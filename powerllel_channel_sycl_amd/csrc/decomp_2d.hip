@@ -64,12 +64,22 @@ void distribute(int num, int nprocs, std::vector<int>& st, std::vector<int>& sz)
   sz[nprocs - 1] = num - st[nprocs - 1] + 1;
 }
 
-// Owner index of global coordinate g (1-based) in a distribution.
-__device__ __forceinline__ int owner(const int* st, int n, int g) {
-  int m = 0;
-  while (m + 1 < n && st[m + 1] <= g) ++m;
-  return m;
+// The last index m < n with t[m] <= key of a non-decreasing table (t[0] <= key):
+// binary search, log2(n) dependent loads instead of up to n.
+template <typename V, typename K>
+__device__ __forceinline__ int last_le(const V* t, int n, K key) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t[mid] <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
+
+// Owner index of global coordinate g (1-based) in a distribution (the last
+// rank whose start is <= g, so ranks of size 0 are passed over).
+__device__ __forceinline__ int owner(const int* st, int n, int g) { return last_le(st, n, g); }
 
 struct XposeArgs {
   int p_row, p_col;
@@ -89,14 +99,33 @@ __global__ __launch_bounds__(256) void k_transpose(XposeArgs a, const int* __res
                                                    const long* __restrict__ off,
                                                    const long* __restrict__ rows_off,
                                                    const int* __restrict__ dist_st,
-                                                   const E* __restrict__ src, E* __restrict__ dst) {
+                                                   const E* __restrict__ src, E* __restrict__ dst,
+                                                   int stage) {
   const int P = a.p_row * a.p_col;
   const int lane = threadIdx.x & 63;
+  // The bookkeeping tables go to LDS once per block (when they fit), so a
+  // row's rank search and metadata reads are LDS reads, and the row's only
+  // global latency is its data.
+  extern __shared__ long xs_lds[];
+  if (stage) {
+    long* s_rows = xs_lds;                                        // P + 1
+    long* s_off = s_rows + (P + 1);                               // 3P
+    int* s_meta = reinterpret_cast<int*>(s_off + 3 * P);          // 18P
+    int* s_dist = s_meta + 18 * P;                                // dist_len
+    for (int t = threadIdx.x; t < P + 1; t += blockDim.x) s_rows[t] = rows_off[t];
+    for (int t = threadIdx.x; t < 3 * P; t += blockDim.x) s_off[t] = off[t];
+    for (int t = threadIdx.x; t < 18 * P; t += blockDim.x) s_meta[t] = meta[t];
+    for (int t = threadIdx.x; t < a.dist_len; t += blockDim.x) s_dist[t] = dist_st[t];
+    __syncthreads();
+    rows_off = s_rows;
+    off = s_off;
+    meta = s_meta;
+    dist_st = s_dist;
+  }
   const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
   for (long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); row < a.rows;
        row += nwaves) {
-    int r = 0;
-    while (r + 1 < P && rows_off[r + 1] <= row) ++r;
+    const int r = last_le(rows_off, P, row);  // destination rank of the row
     const int* dm = meta + (r * 3 + a.pd) * 6;
     const long lr = row - rows_off[r];
     const int lj = (int)(lr % dm[4]);
@@ -104,15 +133,30 @@ __global__ __launch_bounds__(256) void k_transpose(XposeArgs a, const int* __res
     const int gj = dm[1] + lj, gk = dm[2] + lk;
     E* drow = dst + off[r * 3 + a.pd] + lr * dm[3];
     const int c0 = r / a.p_col, c1 = r % a.p_col;
+    // each lane moves U elements per pass: all loads are issued before the stores
+    constexpr int U = 4;
+    const int n = dm[3];
     if (a.split_axis == 0) {
       // y -> x: the x row gathers from p_row source pencils, one run each
-      for (int li = lane; li < dm[3]; li += 64) {
-        const int gi = dm[0] + li;
-        const int m = owner(dist_st, a.dist_len, gi);
-        const int rs = m * a.p_col + c1;
-        const int* sm = meta + (rs * 3 + a.ps) * 6;
-        drow[li] = src[off[rs * 3 + a.ps] + (gi - sm[0]) +
+      for (int l0 = 0; l0 < n; l0 += 64 * U) {
+        E t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int li = l0 + u * 64 + lane;
+          if (li < n) {
+            const int gi = dm[0] + li;
+            const int m = owner(dist_st, a.dist_len, gi);
+            const int rs = m * a.p_col + c1;
+            const int* sm = meta + (rs * 3 + a.ps) * 6;
+            t[u] = src[off[rs * 3 + a.ps] + (gi - sm[0]) +
                        (long)sm[3] * ((gj - sm[1]) + (long)sm[4] * (gk - sm[2]))];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int li = l0 + u * 64 + lane;
+          if (li < n) drow[li] = t[u];
+        }
       }
     } else {
       const int m = owner(dist_st, a.dist_len, a.split_axis == 1 ? gj : gk);
@@ -120,7 +164,27 @@ __global__ __launch_bounds__(256) void k_transpose(XposeArgs a, const int* __res
       const int* sm = meta + (rs * 3 + a.ps) * 6;
       const E* srow = src + off[rs * 3 + a.ps] + (dm[0] - sm[0]) +
                       (long)sm[3] * ((gj - sm[1]) + (long)sm[4] * (gk - sm[2]));
-      for (int li = lane; li < dm[3]; li += 64) drow[li] = srow[li];
+      int l0 = 0;
+      for (; l0 + 64 * U <= n; l0 += 64 * U) {  // whole passes: no guards between the stores
+        E t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) t[u] = srow[l0 + u * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < U; ++u) drow[l0 + u * 64 + lane] = t[u];
+      }
+      if (l0 < n) {  // the rest of the row: guarded, loads still issued together
+        E t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int li = l0 + u * 64 + lane;
+          if (li < n) t[u] = srow[li];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int li = l0 + u * 64 + lane;
+          if (li < n) drow[li] = t[u];
+        }
+      }
     }
   }
 }
@@ -257,12 +321,19 @@ static int transpose(pll_decomp* d, int kind, const void* src, void* dst, int el
   const int* dist = d->d_dists + d->dist_at[kind];
   const long* rows_off = d->d_rows + a.pd * (P + 1);
   const unsigned blocks = (unsigned)std::min<long>((a.rows + 3) / 4, 65536L);
+  // LDS copy of the tables: rows (P+1) and offsets (3P) as long, metadata (18P)
+  // and the distribution as int; staged when it stays small
+  const size_t lds = (size_t)(P + 1 + 3 * P) * sizeof(long) + (size_t)(18 * P + a.dist_len) * sizeof(int);
+  const int stage = lds <= 16384 ? 1 : 0;
+  const size_t shm = stage ? lds : 0;
   if (elem_bytes == 8)
-    hipLaunchKernelGGL((k_transpose<double>), dim3(blocks), dim3(256), 0, 0, a, d->d_meta,
-                       d->d_off, rows_off, dist, static_cast<const double*>(src), static_cast<double*>(dst));
+    hipLaunchKernelGGL((k_transpose<double>), dim3(blocks), dim3(256), shm, 0, a, d->d_meta,
+                       d->d_off, rows_off, dist, static_cast<const double*>(src), static_cast<double*>(dst),
+                       stage);
   else
-    hipLaunchKernelGGL((k_transpose<float>), dim3(blocks), dim3(256), 0, 0, a, d->d_meta,
-                       d->d_off, rows_off, dist, static_cast<const float*>(src), static_cast<float*>(dst));
+    hipLaunchKernelGGL((k_transpose<float>), dim3(blocks), dim3(256), shm, 0, a, d->d_meta,
+                       d->d_off, rows_off, dist, static_cast<const float*>(src), static_cast<float*>(dst),
+                       stage);
   PLL_HIP_TRY(hipGetLastError());
   PLL_HIP_TRY(hipDeviceSynchronize());
   return PLL_OK;
