@@ -383,6 +383,29 @@ __device__ inline void cell_gradient(const T* __restrict__ u, const T* __restric
   gr[8] = (w[a] - w[a - sz]) * dfi;
 }
 
+// The gradients of one input half (see the k-slots of the closure stage):
+// half 0 returns g0..g4 in x[0..4], half 1 g5..g8 in x[0..3] (same formulas
+// and order as cell_gradient).
+template <typename T>
+__device__ inline void cell_gradient_half(const T* __restrict__ u, const T* __restrict__ v,
+                                          const T* __restrict__ w, const Geom& g, long a, T dxi,
+                                          T dyi, T qdxi, T qdyi, T dfi, T hz, int half, T x[5]) {
+  const long sx = g.sx, sz = g.sxy;
+  if (half == 0) {
+    x[0] = (u[a] - u[a - 1]) * dxi;
+    x[1] = ((u[a + sx] + u[a - 1 + sx]) - (u[a - sx] + u[a - 1 - sx])) * qdyi;
+    x[2] = ((u[a + sz] + u[a - 1 + sz]) - (u[a - sz] + u[a - 1 - sz])) * hz;
+    x[3] = ((v[a + 1] + v[a + 1 - sx]) - (v[a - 1] + v[a - 1 - sx])) * qdxi;
+    x[4] = (v[a] - v[a - sx]) * dyi;
+  } else {
+    x[0] = ((v[a + sz] + v[a - sx + sz]) - (v[a - sz] + v[a - sx - sz])) * hz;
+    x[1] = ((w[a + 1] + w[a + 1 - sz]) - (w[a - 1] + w[a - 1 - sz])) * qdxi;
+    x[2] = ((w[a + sx] + w[a + sx - sz]) - (w[a - sx] + w[a - sx - sz])) * qdyi;
+    x[3] = (w[a] - w[a - sz]) * dfi;
+    x[4] = T(0);
+  }
+}
+
 // Network inputs: the gradients times gs, with the four components that
 // change sign under the reflection z -> lz - z (du/dz, dv/dz, dw/dx, dw/dy)
 // multiplied by zs = -1 in the upper half, so both walls look like the lower
@@ -521,6 +544,93 @@ __global__ __launch_bounds__(256) void k_samples(
 // divergence: normal stresses at cell centres, shear stresses at cell edges
 // with nu_t averaged from the four cells around the edge; the wall edges carry
 // no eddy viscosity (the wall stress is the molecular one).
+// ---- register-resident inference (the closure stage) ----------------------
+// The layers are computed transposed, hidden x samples: h1^T = W1^T X^T and
+// h2^T = W2^T h1^T, a 32-sample tile per wave.  The accumulator of a
+// v_mfma_f32_32x32x16_bf16 leaves lane l (half h = l/32) with sample l%32 and
+// the 16 hidden units u(j, h) = 8(j/4) + 4h + j%4 (+32 per row block); the B
+// operand of the next product wants, in lane l, sample l%32 and 8 consecutive
+// k-slots of half h.  Numbering layer 2's k-slots (kb, h, e) as the units
+// u(8(kb%2) + e, h) + 32(kb/2) makes every lane's own results its next
+// operand: no LDS and no barrier between the layers.  The biases start the
+// accumulators; the output dot is split over the halves and joined with one
+// lane exchange.  Same rounding points as forward_tile (BF16 features,
+// weights and activations; FP32 sums); only the summation order differs.
+//
+// Input k-slots: half 0 holds features 0..4 (du/dx, du/dy, du/dz, dv/dx,
+// dv/dy) in slots 0..4, half 1 features 5..8 (dv/dz, dw/dx, dw/dy, dw/dz) in
+// slots 8..11 and the wall distance in slot 12; the other slots are zero.
+__device__ __forceinline__ int hid_unit(int j, int h) { return 8 * (j / 4) + 4 * h + (j % 4); }
+__device__ __forceinline__ int slot_input(int s) {  // feature of input k-slot s, -1: zero
+  return s < 5 ? s : (s < 8 ? -1 : (s < 13 ? s - 3 : -1));
+}
+
+struct RegMlp {
+  bf16x8 a1[2];      // W1^T rows (hidden units l%32 + 32t), k-slots 8h..8h+7
+  bf16x8 a2[2][4];   // W2^T rows (units l%32 + 32t), k-slots of block kb
+  f32x16 c1[2], c2[2];  // biases of the lane's accumulator rows
+  float w3[2][16];   // W3 of the lane's layer-2 rows
+  float b3;
+};
+
+__device__ inline void load_reg_mlp(const Mlp& w, int lane, RegMlp& r) {
+  const int h = lane / 32, m = lane % 32;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int u = m + 32 * t;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int in = slot_input(8 * h + e);
+      r.a1[t][e] = in < 0 ? (short)0 : (short)w.w1[in * kH + u];
+    }
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int unit = hid_unit(8 * (kb % 2) + e, h) + 32 * (kb / 2);
+        r.a2[t][kb][e] = (short)w.w2[unit * kH + u];
+      }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int unit = hid_unit(j, h) + 32 * t;
+      r.c1[t][j] = w.p[oB1 + unit];
+      r.c2[t][j] = w.p[oB2 + unit];
+      r.w3[t][j] = from_bf16(w.w3[unit]);
+    }
+  }
+  r.b3 = w.p[oB3];
+}
+
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  return (uint32_t)to_bf16(lo) | ((uint32_t)to_bf16(hi) << 16);
+}
+
+// o of sample lane%32 (in both halves) from the lane's 8 input slots x.
+__device__ inline float reg_forward(const RegMlp& r, const float x[8]) {
+  bf16x8 b;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) b[e] = (short)to_bf16(x[e]);
+  f32x16 z1[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) z1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.a1[t], b, r.c1[t], 0, 0, 0);
+  bf16x8 b2[4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) b2[kb][e] = (short)to_bf16(fmaxf(z1[kb / 2][8 * (kb % 2) + e], 0.f));
+  float part = 0.f;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    f32x16 z2 = r.c2[t];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+      z2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.a2[t][kb], b2[kb], z2, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) part += r.w3[t][j] * from_bf16(to_bf16(fmaxf(z2[j], 0.f)));
+  }
+  return part + __shfl_xor(part, 32) + r.b3;
+}
+
 constexpr int kLx = 32, kLy = 8, kLz = 4;
 constexpr int kEx = kLx + 2, kEy = kLy + 2, kEz = kLz + 2, kExt = kEx * kEy * kEz;
 constexpr int kLesTiles = (kExt + 31) / 32;
@@ -548,19 +658,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
     T* __restrict__ du, T* __restrict__ dv, T* __restrict__ dw, MeshT<T> m, FeatMesh<T> fm,
     T dxi, T dyi, T qdxi, T qdyi, T rei, T dt, T ucrf, Mlp w, T gs, T ys, float nus,
     float* __restrict__ nut, int ntx, int nty) {
-  __shared__ FwdLds lds[kWaves];
   __shared__ float nu[kExt];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int b = blockIdx.x;
   const int ti = b % ntx, tj = (b / ntx) % nty, tk = b / (ntx * nty);
   const int i0 = ti * kLx, j0 = tj * kLy, k0 = tk * kLz;  // extended cell (0,0,0) = (i0,j0,k0)
-  FwdLds& L = lds[wave];
-  // phase 1: eddy viscosity of the extended block
-  for (int t = wave; t < kLesTiles; t += kWaves) {
-    const int e = t * 32 + lane % 32;
-    bool valid = false;
-    if (lane < 32) {
-      float f[kInP] = {};
+  // phase 1: eddy viscosity of the extended block, every lane of a wave on
+  // one of 32 samples (half of its inputs), the network in registers
+  {
+    RegMlp r;
+    load_reg_mlp(w, lane, r);
+    const int half = lane / 32;
+    for (int t = wave; t < kLesTiles; t += kWaves) {
+      const int e = t * 32 + lane % 32;
+      bool valid = false;
+      float x[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (e < kExt) {
         const int ex = e % kEx, ey = (e / kEx) % kEy, ez = e / (kEx * kEy);
         int i = i0 + ex, j = j0 + ey;
@@ -569,19 +681,23 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
         if (valid) {
           i = i == 0 ? g.nx : (i == g.nx + 1 ? 1 : i);
           j = j == 0 ? g.ny : (j == g.ny + 1 ? 1 : j);
-          T gr[9];
-          cell_gradient(su, sv, sw, g, g.idx(i, j, k), dxi, dyi, qdxi, qdyi, m.dzf_inv[k],
-                        fm.hz[k], gr);
-          features(gr, gs, ys, fm.yw[k], fm.zs[k], f);
+          T gr[5];
+          cell_gradient_half(su, sv, sw, g, g.idx(i, j, k), dxi, dyi, qdxi, qdyi, m.dzf_inv[k],
+                             fm.hz[k], half, gr);
+          // features() of these components: the reflection sign on du/dz,
+          // dv/dz, dw/dx, dw/dy (slots 2 of half 0 and 0..2 of half 1)
+          const T zs = fm.zs[k];
+#pragma unroll
+          for (int c = 0; c < 5; ++c) {
+            T q = gr[c] * gs;
+            if ((half == 0 && c == 2) || (half == 1 && c < 3)) q = q * zs;
+            x[c] = (half == 1 && c == 4) ? (float)(fm.yw[k] * ys) : (float)q;
+          }
         }
       }
-#pragma unroll
-      for (int q = 0; q < kInP; ++q) L.x[lane][q] = to_bf16(f[q]);
+      const float o = reg_forward(r, x);
+      if (lane < 32 && e < kExt) nu[e] = valid ? softplus(o) * nus : 0.f;
     }
-    f32x16 h1[2], h2[2];
-    float o;
-    forward_tile(w, L, lane, h1, h2, o);
-    if (lane < 32 && e < kExt) nu[e] = valid ? softplus(o) * nus : 0.f;
   }
   __syncthreads();
   // phase 2: the stage update of the block's cells
