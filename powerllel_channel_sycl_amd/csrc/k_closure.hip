@@ -637,12 +637,6 @@ constexpr int kLesTiles = (kExt + 31) / 32;
 static_assert(kLesTiles % kWaves == 0, "every wave runs the same number of network tiles");
 static_assert(kLx * kLy == 64 * kWaves, "one thread per (i, j) column of the tile");
 
-struct FwdLds {
-  uint16_t x[32][kInP];
-  uint16_t h1[32][kH];
-  uint16_t h2[32][kH];
-};
-
 template <typename T>
 struct GAcc {
   const T* p;
