@@ -44,7 +44,8 @@ typedef enum pll_status {
   PLL_ERR_HIP = 2,         /* a HIP runtime call failed                     */
   PLL_ERR_STATE = 3,       /* call not valid in the context's current state */
   PLL_ERR_UNSUPPORTED = 4, /* feature not available in this build           */
-  PLL_ERR_NOMEM = 5        /* device allocation failed                      */
+  PLL_ERR_NOMEM = 5,       /* device allocation failed                      */
+  PLL_ERR_IO = 6           /* a file could not be read or written           */
 } pll_status;
 
 typedef enum pll_precision {
@@ -231,6 +232,11 @@ int pll_closure_get_scales(pll_closure *c, double *gscale, double *yscale, doubl
  * src/main.cc:117-153.  Training the closure while attached is ordered
  * before the context's next stage. */
 int pll_closure_attach(pll_context *ctx, pll_closure *c);
+/* Save a closure (parameters and scales) to a file / create one from such a
+ * file: "PLLCLOS1", int32 parameter count, the FP32 parameters, the three FP64
+ * scales, in host byte order. */
+int pll_closure_save(pll_closure *c, const char *path);
+int pll_closure_load(const char *path, pll_closure **out);
 /* nu_t of the context's last closure stage, interior nz*ny*nx, x fastest. */
 int pll_closure_nut(pll_context *ctx, double *host);
 /* Box filter of the fine context's u, v, w onto the coarse context's grid

@@ -116,6 +116,8 @@ _SIGNATURES = {
     "pll_closure_get_scales": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
     "pll_closure_attach": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pll_closure_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pll_closure_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "pll_closure_nut": (C.c_int, [C.c_void_p, _D]),
     "pll_closure_filter": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pll_closure_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -237,6 +239,20 @@ class Closure:
         _check(load_library().pll_closure_get_scales(self._h, C.byref(a), C.byref(b), C.byref(c)),
                "pll_closure_get_scales")
         return a.value, b.value, c.value
+
+    def save(self, path):
+        """Parameters and scales to a file (pll_closure_save)."""
+        _check(load_library().pll_closure_save(self._h, os.fsencode(path)), "pll_closure_save")
+
+    @classmethod
+    def load(cls, path):
+        """A closure with the parameters and scales of a saved file."""
+        obj = cls.__new__(cls)
+        obj._h = C.c_void_p()
+        _check(load_library().pll_closure_load(os.fsencode(path), C.byref(obj._h)),
+               "pll_closure_load")
+        obj.nparam = load_library().pll_closure_num_params()
+        return obj
 
     def samples(self, fine, coarse, x_ptr, y_ptr):
         """Filter fine into coarse and write its n = coarse cells training samples."""

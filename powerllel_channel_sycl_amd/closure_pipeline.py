@@ -118,7 +118,8 @@ def profile_errors(ref, got):
     return {k: float(np.linalg.norm(got[k] - ref[k]) / np.linalg.norm(ref[k])) for k in ref}
 
 
-def run(cfg=Config(), log=say):
+def run(cfg=Config(), log=say, save=None):
+    """The workflow; `save` names a file for the trained closure."""
     import torch
     t_start = time.time()
     c = cfg
@@ -194,6 +195,9 @@ def run(cfg=Config(), log=say):
                                train_loss=loss_on(list(range(min(ntrain, 8)))),
                                adam_steps=c.epochs * ntrain, samples_per_step=ncell,
                                seconds=time.time() - t0)
+        if save:
+            net.save(save)
+            log(f"closure saved to {save}")
         del x, y
         torch.cuda.empty_cache()
         # 4. coarse runs from the filtered DNS state
@@ -249,11 +253,13 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=None, help="write the full result (with profiles) as JSON")
+    ap.add_argument("--save", default=None, help="save the trained closure (pll_closure_save)")
     for f, v in asdict(Config()).items():
         ap.add_argument("--" + f.replace("_", "-"), type=type(v), default=v)
     a = vars(ap.parse_args(argv))
     path = a.pop("out")
-    res = run(Config(**a))
+    save = a.pop("save")
+    res = run(Config(**a), save=save)
     if path:
         with open(path, "w") as fh:
             json.dump(res, fh)

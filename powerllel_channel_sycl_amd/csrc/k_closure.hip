@@ -19,6 +19,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
 #include <random>
 #include <vector>
 
@@ -1000,6 +1003,53 @@ int pll_closure_get_scales(pll_closure* c, double* gscale, double* yscale, doubl
   if (gscale) *gscale = c->gs;
   if (yscale) *yscale = c->ys;
   if (nuscale) *nuscale = c->nus;
+  return PLL_OK;
+}
+
+// File: "PLLCLOS1", int32 parameter count, that many FP32 parameters, then
+// the three FP64 scales (gscale, yscale, nuscale); host byte order.
+static const char kClosureMagic[8] = {'P', 'L', 'L', 'C', 'L', 'O', 'S', '1'};
+
+int pll_closure_save(pll_closure* c, const char* path) {
+  if (!c || !path) return fail(PLL_ERR_ARG, "null argument");
+  std::vector<float> p(kNParam);
+  int rc = pll_closure_get_params(c, p.data());
+  if (rc) return rc;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(PLL_ERR_IO, std::string("cannot write ") + path);
+  const int32_t n = kNParam;
+  const double sc[3] = {c->gs, c->ys, c->nus};
+  const bool ok = std::fwrite(kClosureMagic, 1, 8, f) == 8 && std::fwrite(&n, 4, 1, f) == 1 &&
+                  std::fwrite(p.data(), 4, kNParam, f) == (size_t)kNParam &&
+                  std::fwrite(sc, 8, 3, f) == 3;
+  if (std::fclose(f) != 0 || !ok) return fail(PLL_ERR_IO, std::string("short write to ") + path);
+  return PLL_OK;
+}
+
+int pll_closure_load(const char* path, pll_closure** out) {
+  if (!path || !out) return fail(PLL_ERR_ARG, "null argument");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(PLL_ERR_IO, std::string("cannot read ") + path);
+  char magic[8];
+  int32_t n = 0;
+  std::vector<float> p(kNParam);
+  double sc[3];
+  const bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kClosureMagic, 8) == 0 &&
+                  std::fread(&n, 4, 1, f) == 1 && n == kNParam &&
+                  std::fread(p.data(), 4, kNParam, f) == (size_t)kNParam &&
+                  std::fread(sc, 8, 3, f) == 3;
+  std::fclose(f);
+  if (!ok) return fail(PLL_ERR_IO, std::string(path) + " is not a closure file of this network");
+  pll_closure* c = nullptr;
+  int rc = pll_closure_create(0, &c);
+  if (!rc) rc = pll_closure_set_params(c, p.data());
+  if (!rc) rc = pll_closure_set_scales(c, sc[0], sc[1], sc[2]);
+  if (rc) {
+    pll_closure_destroy(c);
+    return rc;
+  }
+  *out = c;
   return PLL_OK;
 }
 

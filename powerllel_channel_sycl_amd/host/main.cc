@@ -66,8 +66,8 @@ int check_only(const char* fn) {
   std::printf(" \"initial_field\": \"%s\", \"is_forced\": [%d, %d, %d], \"vel_force\": [%.17g, %.17g, %.17g],\n",
               initial_field.c_str(), (int)is_forced[0], (int)is_forced[1], (int)is_forced[2],
               vel_force[0], vel_force[1], vel_force[2]);
-  std::printf(" \"precision\": %d, \"fused_step\": %d, \"dx\": %.17g, \"dy\": %.17g,\n", precision,
-              (int)fused_step, mod_mesh::dx, mod_mesh::dy);
+  std::printf(" \"precision\": %d, \"fused_step\": %d, \"closure\": \"%s\", \"dx\": %.17g, \"dy\": %.17g,\n",
+              precision, (int)fused_step, closure_file.c_str(), mod_mesh::dx, mod_mesh::dy);
   const char* names[] = {"dzf_global", "zc_global", "dzc", "dzflzi"};
   const Array1DH1* arrs[] = {&mod_mesh::dzf_global, &mod_mesh::zc_global, &mod_mesh::dzc,
                              &mod_mesh::dzflzi};
@@ -92,6 +92,10 @@ int run(const char* fn) {
   std::array<std::array<int, 6>, 3> neighbor_xyz{};
   init_poisson_solver(nx, ny, nz, mod_mesh::dx, mod_mesh::dy, mod_mesh::dzf_global.data(), "PP",
                       "PP", "NN", neighbor_xyz);
+  if (!closure_file.empty()) {
+    mod_device::attach_closure(closure_file);
+    std::printf("PowerLLEL.NOTE: Learned closure attached from %s\n", closure_file.c_str());
+  }
 
   int nt_start;
   if (is_restart) {

@@ -5,6 +5,7 @@
 namespace mod_device {
 namespace {
 pll_context* g_ctx = nullptr;
+pll_closure* g_closure = nullptr;
 }
 
 void check(int rc, const char* what) {
@@ -27,7 +28,16 @@ void create(const pll_config& cfg, const double* dzf_global) {
   check(pll_create(&cfg, dzf_global, &g_ctx), "pll_create");
 }
 
+void attach_closure(const std::string& path) {
+  if (g_closure) throw Error(PLL_ERR_STATE, "a closure is already attached");
+  check(pll_closure_load(path.c_str(), &g_closure), "pll_closure_load");
+  check(pll_closure_attach(context(), g_closure), "pll_closure_attach");
+}
+
 void destroy() {
+  if (g_ctx && g_closure) pll_closure_attach(g_ctx, nullptr);
+  if (g_closure) pll_closure_destroy(g_closure);
+  g_closure = nullptr;
   if (g_ctx) pll_destroy(g_ctx);
   g_ctx = nullptr;
 }

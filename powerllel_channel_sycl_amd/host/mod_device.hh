@@ -24,7 +24,9 @@ void check(int rc, const char* what);
 bool has_context();
 pll_context* context();  // throws when none has been created
 void create(const pll_config& cfg, const double* dzf_global);
-void destroy();
+// Load a saved learned closure (pll_closure_load) and attach it to the context.
+void attach_closure(const std::string& path);
+void destroy();  // detaches and frees the closure first
 
 // Synchronous semantics of the reference's module calls: wait for the stream.
 void sync();

@@ -169,6 +169,7 @@ void readInputParam(std::string_view fn_input) {
       bad("precision", "\"fp64\" or \"fp32_mixed\"");
     read(*n, "device", device_id);
     READ(*n, fused_step);
+    read(*n, "closure", closure_file);
   }
   if (!checkMeshAndPara())
     throw std::runtime_error("PowerLLEL.ERROR.checkInputParam: invalid parameters");

@@ -3,7 +3,9 @@
 //
 // Same sections, keys and meaning as the reference's file.  One optional
 // section is added, "DEVICE": {"precision": "fp64" | "fp32_mixed",
-// "device": <HIP ordinal>, "fused_step": true | false}.
+// "device": <HIP ordinal>, "fused_step": true | false, "closure": "<file>"};
+// "closure" names a learned closure saved by pll_closure_save, attached to the
+// run's context (an LES on this grid).
 #pragma once
 
 #include <array>
@@ -84,6 +86,7 @@ inline std::array<int, 6> region_ijk{1, 1, 1, 1, 1, 1};
 inline int precision = PLL_FP64;
 inline int device_id = 0;
 inline bool fused_step = true;
+inline std::string closure_file;  // empty: no closure
 
 // The sanity checks of src/mod_parameters.cc:170-224 plus the one-GPU rule;
 // returns false after printing every failed check.

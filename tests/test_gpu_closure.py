@@ -99,3 +99,30 @@ def test_adam_fits_a_smooth_target():
     # the trained network agrees with the oracle evaluated on its parameters
     _close_to_oracle(out.cpu().numpy(), mo.predict(cl.params(), xt))
     cl.close()
+
+
+def test_save_load_round_trip(tmp_path):
+    import torch
+    print("[closure] save/load", flush=True)
+    cl = pll.Closure(seed=4)
+    p = _random_params(8)
+    cl.set_params(p)
+    cl.set_scales(0.05, 2.5, 1e-4)
+    path = str(tmp_path / "net.closure")
+    cl.save(path)
+    c2 = pll.Closure.load(path)
+    np.testing.assert_array_equal(c2.params(), p)
+    assert c2.scales() == (0.05, 2.5, 1e-4)
+    x, _ = _data(256, 9)
+    dx = _dev(x)
+    a, b = (torch.zeros(256, device="cuda:0") for _ in range(2))
+    cl.predict(dx.data_ptr(), 256, a.data_ptr())
+    c2.predict(dx.data_ptr(), 256, b.data_ptr())
+    cl.synchronize()
+    c2.synchronize()
+    assert torch.equal(a, b)
+    (tmp_path / "bad.closure").write_bytes(b"not a closure")
+    with pytest.raises(pll.PllError, match="not a closure file"):
+        pll.Closure.load(str(tmp_path / "bad.closure"))
+    cl.close()
+    c2.close()

@@ -187,3 +187,24 @@ def test_closure_detached_is_plain_step():
         for f in "uvw":
             np.testing.assert_array_equal(a.download(f), b.download(f))
     cl.close()
+
+
+def test_closure_api_errors():
+    """Bad arguments fail with a status and a message, and leave the objects usable."""
+    cl = pll.Closure(seed=1)
+    for bad in ((0.0, 1.0, 1.0), (1.0, -1.0, 1.0), (1.0, 1.0, float("nan"))):
+        with pytest.raises(pll.PllError):
+            cl.set_scales(*bad)
+    assert cl.scales() == (1.0, 1.0, 1.0)
+    with make_channel("r235") as ch, make_channel("tiny") as other:
+        with pytest.raises(pll.PllError, match="no closure attached"):
+            ch.closure_nut()
+        # grids that do not nest: 8x6x4 is not a coarsening of 24x20x12
+        with pytest.raises(pll.PllError, match="integer multiples|boxes differ|unions"):
+            other.filter_from(ch)
+        with pytest.raises(pll.PllError):
+            cl.samples(ch, other, 0, 0)
+        ch.attach_closure(cl)
+        ch.attach_closure(None)
+        ch.step(1)  # the plain step still runs
+    cl.close()

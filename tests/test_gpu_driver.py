@@ -129,3 +129,21 @@ def test_statistics_files_match_oracle_samples(tmp_path):
     prof = np.loadtxt(str(tmp_path / "stat_00000005-00000010_profiles.txt"))
     assert prof.shape == (16, 19)
     np.testing.assert_allclose(prof[:, 2], sums[0].mean(axis=(1, 2)) / 3, rtol=1e-10)
+
+
+def test_driver_runs_with_a_saved_closure(tmp_path):
+    """DEVICE.closure attaches a saved learned closure to the run's context."""
+    import powerllel_channel_sycl_amd as pll
+    cl = pll.Closure(seed=3)
+    cl.set_scales(0.05, 2.0, 1.0 / 2800.0)
+    cl.save(str(tmp_path / "les.closure"))
+    cl.close()
+    p = du.param(DEVICE={"closure": "les.closure"})
+    r = _run_ok(tmp_path, p)
+    assert "Learned closure attached from les.closure" in r.stdout
+    lines = [l.split() for l in r.stdout.splitlines() if l.strip().startswith(("5 ", "10 "))]
+    assert len(lines) == 2 and all(float(l[-1]) < 1e-12 for l in lines), r.stdout
+    # the eddy viscosity changes the flow: the run differs from the plain one
+    (ou, _, _), _ = _oracle_fields(p, 10)
+    nt, got = _read(str(tmp_path / "save_u.h5"), "u")
+    assert nt == 10 and _rel(got, ou[1:-1, 1:-1, 1:-1]) > 1e-9

@@ -44,10 +44,11 @@ def test_read_mesh_from_mesh_in(tmp_path):
 
 
 def test_device_section(tmp_path):
-    r = _check(tmp_path, du.param(DEVICE={"precision": "fp32_mixed", "fused_step": False}))
+    r = _check(tmp_path, du.param(DEVICE={"precision": "fp32_mixed", "fused_step": False,
+                                          "closure": "les.closure"}))
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
-    assert out["precision"] == 1 and out["fused_step"] == 0
+    assert out["precision"] == 1 and out["fused_step"] == 0 and out["closure"] == "les.closure"
 
 
 @pytest.mark.parametrize("sections, message", [
