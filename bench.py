@@ -60,11 +60,15 @@ STEP_BYTES_FP64 = 312
 # profiles/retau550_768x384x384_step_kernels.md holds the runs.  Reported as
 # the roofline entry's `traffic` for that workload only.
 MEASURED_HBM_B_PER_POINT = {
-    "fp64": {"rk1": 51.2, "rk2_rhs": 87.7, "fft_x_fwd": 16.1, "fft_y_fwd": 16.0, "solve_trid": 38.7,
+    "fp64": {"rk1": 51.2, "rk2_rhs": 87.8, "fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 38.7,
              "fft_y_bwd": 16.0, "fft_x_bwd": 16.0, "correct_vel": 60.9},
-    "fp32_mixed": {"rk1": 27.0, "rk2": 38.9, "fft_x_fwd_rhs": 21.0, "fft_y_fwd": 8.0,
+    "fp32_mixed": {"rk1": 27.1, "rk2": 38.9, "fft_x_fwd_rhs": 20.8, "fft_y_fwd": 8.0,
                    "solve_trid": 22.7, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
 }
+
+# MFMA flops per training sample of the closure network (forward 2(16*64 +
+# 64*64), backward dW2, delta1 and dW1: 2(64*64 + 64*64 + 16*64)).
+CLOSURE_TRAIN_FLOP = 10240 + 18432
 
 
 def parse():
@@ -75,11 +79,69 @@ def parse():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--fp64-only", action="store_true", help="skip the FP32-mixed measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the learned-closure measurement (coarse 96x64x64 grid)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of 1 GiB (bandwidth ceiling and the "
                          "scale factor of the FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def closure_bench(steps=50, warmup=5):
+    """The learned closure on the coarse grid of the Re_tau~395 closure
+    workflow (96 x 64 x 64, 2pi x pi x 2, stretch 1.5, re = 6875, dt = 0.008,
+    FP64): the step with and without a closure attached (its network
+    evaluated inside the RK stage kernels), and Adam steps of the network's
+    MFMA training kernels on one snapshot's worth of samples (393,216)."""
+    import torch  # device buffers for the samples; imported before libpll.so
+    import numpy as np
+
+    import powerllel_channel_sycl_amd as pll
+    nx, ny, nz, re = 96, 64, 64, 6875.0
+    dzf = pll.mesh_by_func(1, 0, 1.5, nz, 2.0)
+    utau = 395.0 / re
+    cl = pll.Closure(seed=7)
+    cl.set_scales(1.0 / (re * utau * utau), utau * re / 100.0, 1.0 / re)
+    out = {"workload": "closure coarse 96x64x64 (Re_tau~395), FP64 step"}
+    for name in ("plain", "closure"):
+        with pll.Channel(nx, ny, nz, 2 * np.pi, np.pi, 2.0, re, 0.008, dzf) as ch:
+            ch.initFlow("log", noise_intensity=0.1)
+            if name == "closure":
+                ch.attach_closure(cl)
+            ch.step(warmup)
+            ch.synchronize()
+            t0 = time.perf_counter()
+            ch.step(steps)
+            ch.synchronize()
+            ms = (time.perf_counter() - t0) / steps * 1e3
+            ch.timing(True)
+            ch.timing_reset()
+            ch.step(10)
+            ch.synchronize()
+            t = ch.timing_read()
+            stage = t["rk_closure" if name == "closure" else "rk1"]
+            out[name] = {"ms_per_step": round(ms, 4),
+                         "stage_us": round(stage[0] / max(stage[1], 1) * 1e3, 1)}
+            if name == "closure":
+                ch.attach_closure(None)
+    n = nx * ny * nz
+    g = torch.Generator(device="cuda").manual_seed(1)
+    x = torch.randn((n, 10), device="cuda", generator=g)
+    y = torch.rand((n,), device="cuda", generator=g)
+    for _ in range(3):
+        cl.train_step(x.data_ptr(), y.data_ptr(), n, 1e-3)
+    cl.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cl.train_step(x.data_ptr(), y.data_ptr(), n, 1e-3)
+    cl.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    out["training"] = {"samples_per_adam_step": n, "us_per_adam_step": round(sec * 1e6, 1),
+                       "samples_per_s": round(n / sec), "mfma_tflops": round(n * CLOSURE_TRAIN_FLOP / sec / 1e12, 1)}
+    cl.close()
+    return out
 
 
 def copy_probe():
@@ -202,6 +264,10 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(sharding across GPUs is designed, not built)"}))
         sys.exit(2)
+    if not args.no_closure:
+        # torch (device buffers of the closure's training samples) brings its
+        # own HIP runtime, which must be loaded before libpll.so binds one
+        import torch  # noqa: F401
     main64 = measure(args.workload, "fp64", args.steps, args.warmup)
     result = {
         "metric": "Mpoint-steps/s per RK2 step (FP64 vs FP32-mixed) + % HBM roofline, 1 GPU",
@@ -230,6 +296,8 @@ def main():
             "speedup_vs_fp64": round(mixed["value"] / main64["value"], 3),
             "roofline": mixed["roofline"], "kernels": mixed["kernels"], "check": mixed["check"],
         }
+    if not args.no_closure:
+        result["closure"] = closure_bench()
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", flush=True)
         result["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps)
