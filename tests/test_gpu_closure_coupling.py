@@ -104,13 +104,15 @@ def _closure_with_params(seed, re):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
-@pytest.mark.parametrize("name", ["r235", "gen"])
-def test_closure_stages_match_oracle(name, precision):
-    print(f"CASE closure stages {name} {precision}", flush=True)
+@pytest.mark.parametrize("name,crf", [("r235", 0.0), ("gen", 0.0), ("r235", 0.3)])
+def test_closure_stages_match_oracle(name, crf, precision):
+    print(f"CASE closure stages {name} u_crf={crf} {precision}", flush=True)
     prm = params(name)
-    o = make_oracle(name)
+    kw = dict(u_crf=crf) if crf else {}
+    o = make_oracle(name, **kw)
     u, v, w, u1, v1, w1 = (o.field() for _ in range(6))
     o.init_flow(u, v, w, "poi", noise_intensity=0.2)
+    u -= crf  # the state in the moving frame
     o.update_bound_vel(u, v, w)
     cl, p = _closure_with_params(11, prm["re"])
     gs, ys, nus = cl.scales()
@@ -118,7 +120,7 @@ def test_closure_stages_match_oracle(name, precision):
     hz, yw, zs = mo.feat_mesh(mesh, prm["lz"])
     dx, dy = prm["lx"] / prm["nx"], prm["ly"] / prm["ny"]
     dt = prm["dt"]
-    with make_channel(name, precision=precision) as ch:
+    with make_channel(name, precision=precision, **kw) as ch:
         for f, a in zip("uvw", (u, v, w)):
             ch.upload(f, a)
         U, V, W = (ch.download(f) for f in "uvw")
