@@ -53,7 +53,20 @@ __host__ __device__ inline float from_bf16(uint16_t b) {
   return x;
 }
 
-__device__ inline float softplus(float o) { return o > 20.f ? o : log1pf(expf(o)); }
+// softplus(o) = log(1 + e^o) = max(o, 0) + log1p(t), t = e^-|o| in (0, 1],
+// with log1p(t) in Goldberg's form log(u) t / (u - 1), u = 1 + t rounded,
+// which keeps the relative accuracy of the logarithm for small t (D. Goldberg,
+// "What every computer scientist should know about floating-point
+// arithmetic", 1991, Theorem 4); the exponential, logarithm and quotient on the
+// hardware exp2/log2/reciprocal (relative error ~1e-6 in all; u lies in
+// [1, 2], so no denormal scaling is needed).
+__device__ inline float softplus(float o) {
+  constexpr float kLog2e = 1.44269504088896341f, kLn2 = 0.693147180559945309f;
+  const float t = __builtin_amdgcn_exp2f(-fabsf(o) * kLog2e);
+  const float u = 1.f + t;
+  const float l = u == 1.f ? t : (__builtin_amdgcn_logf(u) * kLn2) * (t * __builtin_amdgcn_rcpf(u - 1.f));
+  return fmaxf(o, 0.f) + l;
+}
 __device__ inline float sigmoid(float o) { return 1.f / (1.f + expf(-o)); }
 
 struct Mlp {
@@ -568,11 +581,28 @@ __device__ __forceinline__ int slot_input(int s) {  // feature of input k-slot s
   return s < 5 ? s : (s < 8 ? -1 : (s < 13 ? s - 3 : -1));
 }
 
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Two FP32 values rounded to BF16 (nearest, ties to even) in one dword, lo | hi << 16.
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+// BF16 of relu(lo), relu(hi), packed: a negative value rounds to a BF16 with
+// its sign bit set, which a 16-bit signed max against 0 clears to +0, so this
+// is the BF16 of max(x, 0) in two instructions for the pair.
+__device__ __forceinline__ uint32_t pack_relu_bf16(float lo, float hi) {
+  const s16x2 q = __builtin_bit_cast(s16x2, pack_bf16(lo, hi));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(q, (s16x2){0, 0}));
+}
+
 struct RegMlp {
   bf16x8 a1[2];      // W1^T rows (hidden units l%32 + 32t), k-slots 8h..8h+7
   bf16x8 a2[2][4];   // W2^T rows (units l%32 + 32t), k-slots of block kb
   f32x16 c1[2], c2[2];  // biases of the lane's accumulator rows
-  float w3[2][16];   // W3 of the lane's layer-2 rows
+  uint32_t w3[2][8];    // BF16 W3 of the lane's layer-2 rows j = 2m, 2m+1, packed
   float b3;
 };
 
@@ -598,39 +628,48 @@ __device__ inline void load_reg_mlp(const Mlp& w, int lane, RegMlp& r) {
       const int unit = hid_unit(j, h) + 32 * t;
       r.c1[t][j] = w.p[oB1 + unit];
       r.c2[t][j] = w.p[oB2 + unit];
-      r.w3[t][j] = from_bf16(w.w3[unit]);
     }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      r.w3[t][q] = (uint32_t)w.w3[hid_unit(2 * q, h) + 32 * t] |
+                   ((uint32_t)w.w3[hid_unit(2 * q + 1, h) + 32 * t] << 16);
   }
   r.b3 = w.p[oB3];
 }
 
-__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
-  return (uint32_t)to_bf16(lo) | ((uint32_t)to_bf16(hi) << 16);
-}
-
-// o of sample lane%32 (in both halves) from the lane's 8 input slots x.
+// o of sample lane%32 (in both halves) from the lane's 8 input slots x.  The
+// output dot runs on v_dot2c_f32_bf16 (BF16 h2 times BF16 W3, FP32 sum).
 __device__ inline float reg_forward(const RegMlp& r, const float x[8]) {
-  bf16x8 b;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) b[e] = (short)to_bf16(x[e]);
+  const bf16x8 b = __builtin_bit_cast(
+      bf16x8, (u32x4){pack_bf16(x[0], x[1]), pack_bf16(x[2], x[3]), pack_bf16(x[4], x[5]),
+                      pack_bf16(x[6], x[7])});
   f32x16 z1[2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) z1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.a1[t], b, r.c1[t], 0, 0, 0);
   bf16x8 b2[4];
 #pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    const f32x16& z = z1[kb / 2];
+    const int o = 8 * (kb % 2);
+    b2[kb] = __builtin_bit_cast(
+        bf16x8, (u32x4){pack_relu_bf16(z[o], z[o + 1]), pack_relu_bf16(z[o + 2], z[o + 3]),
+                        pack_relu_bf16(z[o + 4], z[o + 5]), pack_relu_bf16(z[o + 6], z[o + 7])});
+  }
+  // the two row blocks' products alternate, so consecutive MFMAs are independent
+  f32x16 z2[2] = {r.c2[0], r.c2[1]};
+#pragma unroll
   for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) b2[kb][e] = (short)to_bf16(fmaxf(z1[kb / 2][8 * (kb % 2) + e], 0.f));
+    for (int t = 0; t < 2; ++t)
+      z2[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.a2[t][kb], b2[kb], z2[t], 0, 0, 0);
   float part = 0.f;
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    f32x16 z2 = r.c2[t];
+  for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
-      z2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.a2[t][kb], b2[kb], z2, 0, 0, 0);
-#pragma unroll
-    for (int j = 0; j < 16; ++j) part += r.w3[t][j] * from_bf16(to_bf16(fmaxf(z2[j], 0.f)));
-  }
+    for (int q = 0; q < 8; ++q)
+      part = __builtin_amdgcn_fdot2_f32_bf16(
+          __builtin_bit_cast(bf16x2, pack_relu_bf16(z2[t][2 * q], z2[t][2 * q + 1])),
+          __builtin_bit_cast(bf16x2, r.w3[t][q]), part, false);
   return part + __shfl_xor(part, 32) + r.b3;
 }
 
