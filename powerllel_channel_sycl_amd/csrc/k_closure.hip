@@ -399,29 +399,6 @@ __device__ inline void cell_gradient(const T* __restrict__ u, const T* __restric
   gr[8] = (w[a] - w[a - sz]) * dfi;
 }
 
-// The gradients of one input half (see the k-slots of the closure stage):
-// half 0 returns g0..g4 in x[0..4], half 1 g5..g8 in x[0..3] (same formulas
-// and order as cell_gradient).
-template <typename T>
-__device__ inline void cell_gradient_half(const T* __restrict__ u, const T* __restrict__ v,
-                                          const T* __restrict__ w, const Geom& g, long a, T dxi,
-                                          T dyi, T qdxi, T qdyi, T dfi, T hz, int half, T x[5]) {
-  const long sx = g.sx, sz = g.sxy;
-  if (half == 0) {
-    x[0] = (u[a] - u[a - 1]) * dxi;
-    x[1] = ((u[a + sx] + u[a - 1 + sx]) - (u[a - sx] + u[a - 1 - sx])) * qdyi;
-    x[2] = ((u[a + sz] + u[a - 1 + sz]) - (u[a - sz] + u[a - 1 - sz])) * hz;
-    x[3] = ((v[a + 1] + v[a + 1 - sx]) - (v[a - 1] + v[a - 1 - sx])) * qdxi;
-    x[4] = (v[a] - v[a - sx]) * dyi;
-  } else {
-    x[0] = ((v[a + sz] + v[a - sx + sz]) - (v[a - sz] + v[a - sx - sz])) * hz;
-    x[1] = ((w[a + 1] + w[a + 1 - sz]) - (w[a - 1] + w[a - 1 - sz])) * qdxi;
-    x[2] = ((w[a + sx] + w[a + sx - sz]) - (w[a - sx] + w[a - sx - sz])) * qdyi;
-    x[3] = (w[a] - w[a - sz]) * dfi;
-    x[4] = T(0);
-  }
-}
-
 // Network inputs: the gradients times gs, with the four components that
 // change sign under the reflection z -> lz - z (du/dz, dv/dz, dw/dx, dw/dy)
 // multiplied by zs = -1 in the upper half, so both walls look like the lower
@@ -688,6 +665,38 @@ struct GAcc {
   }
 };
 
+// The 16 field values one half of a sample's inputs reads, as (field, offset
+// from the cell a = idx(i,j,k)).  Both halves take three 4-point central
+// differences ((p0 + p1) - (p2 + p3), Q0..Q2) and two face differences
+// (p0 - p1, D0 and D1), in cell_gradient's operand order:
+//   half 0: Q0 du/dy, Q1 du/dz, Q2 dv/dx, D0 du/dx, D1 dv/dy
+//   half 1: Q0 dv/dz, Q1 dw/dx, Q2 dw/dy, D0 dw/dz (D1 unused, reads w[a], w[a-sz])
+// so the two halves of a wave run the same instructions on different addresses.
+template <typename T>
+__device__ inline void les_half_loads(const T* u, const T* v, const T* w, long sx, long sz,
+                                      int half, const T* fp[16], long fo[16]) {
+  const long o0[16] = {sx, sx - 1, -sx, -sx - 1, sz, sz - 1, -sz, -sz - 1,
+                       1,  1 - sx, -1,  -1 - sx, 0,  -1,     0,   -sx};
+  const long o1[16] = {sz, sz - sx, -sz, -sz - sx, 1,  1 - sz, -1,  -1 - sz,
+                       sx, sx - sz, -sx, -sx - sz, 0,  -sz,    0,   -sz};
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const T* f0 = (s < 8 || s == 12 || s == 13) ? u : v;
+    const T* f1 = s < 4 ? v : w;
+    fp[s] = half ? f1 : f0;
+    fo[s] = half ? o1[s] : o0[s];
+  }
+}
+
+// One tile's inputs for a lane: the fetched field values, the plane
+// coefficients of its cell, and whether the sample is a cell of the domain.
+template <typename T>
+struct LesIn {
+  T v[16];
+  T hz, dfi, zs, yw;
+  bool valid;
+};
+
 template <typename T, int STAGE>
 __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
     Geom g, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
@@ -700,37 +709,62 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
   const int ti = b % ntx, tj = (b / ntx) % nty, tk = b / (ntx * nty);
   const int i0 = ti * kLx, j0 = tj * kLy, k0 = tk * kLz;  // extended cell (0,0,0) = (i0,j0,k0)
   // phase 1: eddy viscosity of the extended block, every lane of a wave on
-  // one of 32 samples (half of its inputs), the network in registers
+  // one of 32 samples (half of its inputs), the network in registers; the
+  // field values of a wave's next tile are loaded before its current tile's
+  // network runs
   {
     RegMlp r;
     load_reg_mlp(w, lane, r);
     const int half = lane / 32;
-    for (int t = wave; t < kLesTiles; t += kWaves) {
-      const int e = t * 32 + lane % 32;
-      bool valid = false;
-      float x[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (e < kExt) {
-        const int ex = e % kEx, ey = (e / kEx) % kEy, ez = e / (kEx * kEy);
-        int i = i0 + ex, j = j0 + ey;
-        const int k = k0 + ez;
-        valid = i <= g.nx + 1 && j <= g.ny + 1 && k >= 1 && k <= g.nz;
-        if (valid) {
-          i = i == 0 ? g.nx : (i == g.nx + 1 ? 1 : i);
-          j = j == 0 ? g.ny : (j == g.ny + 1 ? 1 : j);
-          T gr[5];
-          cell_gradient_half(su, sv, sw, g, g.idx(i, j, k), dxi, dyi, qdxi, qdyi, m.dzf_inv[k],
-                             fm.hz[k], half, gr);
-          // features() of these components: the reflection sign on du/dz,
-          // dv/dz, dw/dx, dw/dy (slots 2 of half 0 and 0..2 of half 1)
-          const T zs = fm.zs[k];
+    const T* fp[16];
+    long fo[16];
+    les_half_loads(su, sv, sw, g.sx, g.sxy, half, fp, fo);
+    LesIn<T> in;
+    auto fetch = [&](int t) {
+      const int e0 = t * 32 + lane % 32;
+      const int e = e0 < kExt ? e0 : kExt - 1;
+      const int ex = e % kEx, ey = (e / kEx) % kEy, ez = e / (kEx * kEy);
+      int i = i0 + ex, j = j0 + ey, k = k0 + ez;
+      in.valid = e0 < kExt && i <= g.nx + 1 && j <= g.ny + 1 && k >= 1 && k <= g.nz;
+      // periodic images in x and y; a sample outside the domain reads some
+      // interior cell instead and is discarded
+      i = i == 0 ? g.nx : (i > g.nx ? 1 : i);
+      j = j == 0 ? g.ny : (j > g.ny ? 1 : j);
+      k = k < 1 ? 1 : (k > g.nz ? g.nz : k);
+      const long a = g.idx(i, j, k);
 #pragma unroll
-          for (int c = 0; c < 5; ++c) {
-            T q = gr[c] * gs;
-            if ((half == 0 && c == 2) || (half == 1 && c < 3)) q = q * zs;
-            x[c] = (half == 1 && c == 4) ? (float)(fm.yw[k] * ys) : (float)q;
-          }
-        }
-      }
+      for (int s = 0; s < 16; ++s) in.v[s] = fp[s][a + fo[s]];
+      in.hz = fm.hz[k];
+      in.dfi = m.dzf_inv[k];
+      in.zs = fm.zs[k];
+      in.yw = fm.yw[k];
+    };
+    fetch(wave);
+    for (int t = wave; t < kLesTiles; t += kWaves) {
+      // cell_gradient's differences and features()' scaling and reflection
+      // signs (du/dz, dv/dz, dw/dx, dw/dy); multiplying by 1 is exact
+      const T one = T(1);
+      const T q0 = ((in.v[0] + in.v[1]) - (in.v[2] + in.v[3])) * (half ? in.hz : qdyi);
+      const T q1 = ((in.v[4] + in.v[5]) - (in.v[6] + in.v[7])) * (half ? qdxi : in.hz);
+      const T q2 = ((in.v[8] + in.v[9]) - (in.v[10] + in.v[11])) * (half ? qdyi : qdxi);
+      const T d0 = (in.v[12] - in.v[13]) * (half ? in.dfi : dxi);
+      const T d1 = (in.v[14] - in.v[15]) * dyi;
+      const float fq0 = (float)((q0 * gs) * (half ? in.zs : one));
+      const float fq1 = (float)((q1 * gs) * in.zs);
+      const float fq2 = (float)((q2 * gs) * (half ? in.zs : one));
+      const float fd0 = (float)(d0 * gs);
+      const float fd1 = half ? (float)(in.yw * ys) : (float)(d1 * gs);
+      const bool valid = in.valid;
+      // input slots: half 0 (du/dx, du/dy, du/dz, dv/dx, dv/dy), half 1
+      // (dv/dz, dw/dx, dw/dy, dw/dz, wall distance)
+      float x[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      x[0] = valid ? (half ? fq0 : fd0) : 0.f;
+      x[1] = valid ? (half ? fq1 : fq0) : 0.f;
+      x[2] = valid ? (half ? fq2 : fq1) : 0.f;
+      x[3] = valid ? (half ? fd0 : fq2) : 0.f;
+      x[4] = valid ? fd1 : 0.f;
+      if (t + kWaves < kLesTiles) fetch(t + kWaves);
+      const int e = t * 32 + lane % 32;
       const float o = reg_forward(r, x);
       if (lane < 32 && e < kExt) nu[e] = valid ? softplus(o) * nus : 0.f;
     }
