@@ -656,15 +656,6 @@ constexpr int kLesTiles = (kExt + 31) / 32;
 static_assert(kLesTiles % kWaves == 0, "every wave runs the same number of network tiles");
 static_assert(kLx * kLy == 64 * kWaves, "one thread per (i, j) column of the tile");
 
-template <typename T>
-struct GAcc {
-  const T* p;
-  long sx, sxy;
-  __device__ __forceinline__ T operator()(int di, int dj, int dk) const {
-    return p[di + dj * sx + dk * sxy];
-  }
-};
-
 // The 16 field values one half of a sample's inputs reads, as (field, offset
 // from the cell a = idx(i,j,k)).  Both halves take three 4-point central
 // differences ((p0 + p1) - (p2 + p3), Q0..Q2) and two face differences
@@ -687,6 +678,64 @@ __device__ inline void les_half_loads(const T* u, const T* v, const T* w, long s
     fo[s] = half ? o1[s] : o0[s];
   }
 }
+
+// The values of one field that the closure stage reads around a cell: slot 0
+// the cell, 1..6 its face neighbours (+x, -x, +y, -y, +z, -z), 7 and 8 two
+// diagonal neighbours, which differ by field (u: (-1,1,0), (-1,0,1); v:
+// (1,-1,0), (0,-1,1); w: (1,0,-1), (0,1,-1)).  Every offset tendency() and the
+// stress divergence use is one of these.
+constexpr int kStencil[3][9][3] = {
+    {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}, {-1, 1, 0}, {-1, 0, 1}},
+    {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}, {1, -1, 0}, {0, -1, 1}},
+    {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}, {1, 0, -1}, {0, 1, -1}}};
+
+__host__ __device__ constexpr int stencil_slot(int f, int di, int dj, int dk) {
+  return (di == 0 && dj == 0 && dk == 0)   ? 0
+         : (dj == 0 && dk == 0)            ? (di == 1 ? 1 : 2)
+         : (di == 0 && dk == 0)            ? (dj == 1 ? 3 : 4)
+         : (di == 0 && dj == 0)            ? (dk == 1 ? 5 : 6)
+         : (f == 2 ? (dj == 0 ? 7 : 8) : (dk == 0 ? 7 : 8));
+}
+static_assert(stencil_slot(0, -1, 1, 0) == 7 && stencil_slot(0, -1, 0, 1) == 8 &&
+                  stencil_slot(1, 1, -1, 0) == 7 && stencil_slot(1, 0, -1, 1) == 8 &&
+                  stencil_slot(2, 1, 0, -1) == 7 && stencil_slot(2, 0, 1, -1) == 8 &&
+                  stencil_slot(1, 0, 0, -1) == 6 && stencil_slot(2, -1, 0, 0) == 2,
+              "stencil slots follow kStencil");
+
+// One plane's stencil values of the three fields (and, for stage 2, the old
+// values of the output arrays at the cell), held in registers.
+template <typename T>
+struct LesPlane {
+  T f[3][9];
+  T o[3];
+};
+
+template <typename T, bool OLD>
+__device__ __forceinline__ void load_les_plane(LesPlane<T>& p, const T* u, const T* v, const T* w,
+                                               const T* ou, const T* ov, const T* ow, long a,
+                                               long sx, long sz) {
+  const T* src[3] = {u, v, w};
+#pragma unroll
+  for (int f = 0; f < 3; ++f)
+#pragma unroll
+    for (int s = 0; s < 9; ++s)
+      p.f[f][s] = src[f][a + kStencil[f][s][0] + kStencil[f][s][1] * sx + kStencil[f][s][2] * sz];
+  if (OLD) {
+    p.o[0] = ou[a];
+    p.o[1] = ov[a];
+    p.o[2] = ow[a];
+  }
+}
+
+// tendency()'s stencil accessor over a LesPlane field (f = 0, 1, 2 for u, v, w).
+template <typename T>
+struct RegAcc {
+  const T* r;
+  int f;
+  __device__ __forceinline__ T operator()(int di, int dj, int dk) const {
+    return r[stencil_slot(f, di, dj, dk)];
+  }
+};
 
 // One tile's inputs for a lane: the fetched field values, the plane
 // coefficients of its cell, and whether the sample is a cell of the domain.
@@ -770,16 +819,26 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
     }
   }
   __syncthreads();
-  // phase 2: the stage update of the block's cells
+  // phase 2: the stage update of the block's cells, a column of kLz planes
+  // per thread; the stencil values of the next plane are loaded before the
+  // current plane is computed.  Planes beyond nz (a ragged last block) are
+  // computed on plane nz's values and not stored.
   const int tx = threadIdx.x % kLx, ty = threadIdx.x / kLx;
   const int i = i0 + tx + 1, j = j0 + ty + 1;
   if (i > g.nx || j > g.ny) return;
   const T two = T(2), qtr = T(0.25);
+  auto plane_of = [&](int kz) { const int k = k0 + kz + 1; return k < g.nz ? k : g.nz; };
+  LesPlane<T> cur, nxt;
+  load_les_plane<T, STAGE == 2>(cur, su, sv, sw, du, dv, dw, g.idx(i, j, plane_of(0)), g.sx, g.sxy);
+#pragma unroll
   for (int kz = 0; kz < kLz; ++kz) {
-    const int k = k0 + kz + 1;
-    if (k > g.nz) break;
+    const int k = plane_of(kz);
+    const bool live = k0 + kz + 1 <= g.nz;
+    if (kz + 1 < kLz)
+      load_les_plane<T, STAGE == 2>(nxt, su, sv, sw, du, dv, dw, g.idx(i, j, plane_of(kz + 1)),
+                                    g.sx, g.sxy);
     const long a = g.idx(i, j, k);
-    const GAcc<T> U{su + a, g.sx, g.sxy}, V{sv + a, g.sx, g.sxy}, W{sw + a, g.sx, g.sxy};
+    const RegAcc<T> U{cur.f[0], 0}, V{cur.f[1], 1}, W{cur.f[2], 2};
     T tu, tv, tw;
     tendency<T>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw);
     const int c = (tx + 1) + kEx * ((ty + 1) + kEy * (kz + 1));
@@ -795,8 +854,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
       const T gN = (U(0, 1, 0) - U(0, 0, 0)) * dyi + (V(1, 0, 0) - V(0, 0, 0)) * dxi;
       const T gS = (U(0, 0, 0) - U(0, -1, 0)) * dyi + (V(1, -1, 0) - V(0, -1, 0)) * dxi;
       const T fxy = (nN * gN - nS * gS) * dyi;
-      const T nT = top ? T(0) : qtr * (((N(0, 0, 0) + N(1, 0, 0)) + N(0, 0, 1)) + N(1, 0, 1));
-      const T nB = bot ? T(0) : qtr * (((N(0, 0, -1) + N(1, 0, -1)) + N(0, 0, 0)) + N(1, 0, 0));
+      // the wall edges carry no eddy viscosity (all neighbours are in the
+      // block's LDS, so both sides are computed and one is selected)
+      const T nTf = qtr * (((N(0, 0, 0) + N(1, 0, 0)) + N(0, 0, 1)) + N(1, 0, 1));
+      const T nBf = qtr * (((N(0, 0, -1) + N(1, 0, -1)) + N(0, 0, 0)) + N(1, 0, 0));
+      const T nT = top ? T(0) : nTf, nB = bot ? T(0) : nBf;
       const T gT = (U(0, 0, 1) - U(0, 0, 0)) * dci + (W(1, 0, 0) - W(0, 0, 0)) * dxi;
       const T gB = (U(0, 0, 0) - U(0, 0, -1)) * dcm + (W(1, 0, -1) - W(0, 0, -1)) * dxi;
       const T fxz = (nT * gT - nB * gB) * dfi;
@@ -810,16 +872,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
       const T fyx = (nE * gE - nW * gW) * dxi;
       const T sN = (V(0, 1, 0) - V(0, 0, 0)) * dyi, sS = (V(0, 0, 0) - V(0, -1, 0)) * dyi;
       const T fyy = ((N(0, 1, 0) * sN - N(0, 0, 0) * sS) * two) * dyi;
-      const T nT = top ? T(0) : qtr * (((N(0, 0, 0) + N(0, 1, 0)) + N(0, 0, 1)) + N(0, 1, 1));
-      const T nB = bot ? T(0) : qtr * (((N(0, 0, -1) + N(0, 1, -1)) + N(0, 0, 0)) + N(0, 1, 0));
+      const T nTf = qtr * (((N(0, 0, 0) + N(0, 1, 0)) + N(0, 0, 1)) + N(0, 1, 1));
+      const T nBf = qtr * (((N(0, 0, -1) + N(0, 1, -1)) + N(0, 0, 0)) + N(0, 1, 0));
+      const T nT = top ? T(0) : nTf, nB = bot ? T(0) : nBf;
       const T gT = (V(0, 0, 1) - V(0, 0, 0)) * dci + (W(0, 1, 0) - W(0, 0, 0)) * dyi;
       const T gB = (V(0, 0, 0) - V(0, 0, -1)) * dcm + (W(0, 1, -1) - W(0, 0, -1)) * dyi;
       const T fyz = (nT * gT - nB * gB) * dfi;
       s_v = (fyx + fyy) + fyz;
     }
-    if (top) {
-      s_w = T(0);  // w(nz) is the wall value
-    } else {  // z-momentum at the w face (i, j, k+1/2)
+    {  // z-momentum at the w face (i, j, k+1/2); w(nz) is the wall value
       const T nE = qtr * (((N(0, 0, 0) + N(1, 0, 0)) + N(0, 0, 1)) + N(1, 0, 1));
       const T nW = qtr * (((N(-1, 0, 0) + N(0, 0, 0)) + N(-1, 0, 1)) + N(0, 0, 1));
       const T gE = (U(0, 0, 1) - U(0, 0, 0)) * dci + (W(1, 0, 0) - W(0, 0, 0)) * dxi;
@@ -833,18 +894,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
       const T sT = (W(0, 0, 1) - W(0, 0, 0)) * m.dzf_inv[k + 1];
       const T sB = (W(0, 0, 0) - W(0, 0, -1)) * dfi;
       const T fzz = ((N(0, 0, 1) * sT - N(0, 0, 0) * sB) * two) * dci;
-      s_w = (fzx + fzy) + fzz;
+      s_w = top ? T(0) : (fzx + fzy) + fzz;
     }
-    if (STAGE == 1) {
-      du[a] = U(0, 0, 0) + dt * tu + dt * s_u;
-      dv[a] = V(0, 0, 0) + dt * tv + dt * s_v;
-      dw[a] = W(0, 0, 0) + dt * tw + dt * s_w;
-    } else {
-      du[a] = (U(0, 0, 0) + dt * tu + du[a] + dt * s_u) * T(0.5);
-      dv[a] = (V(0, 0, 0) + dt * tv + dv[a] + dt * s_v) * T(0.5);
-      dw[a] = (W(0, 0, 0) + dt * tw + dw[a] + dt * s_w) * T(0.5);
+    if (live) {
+      if (STAGE == 1) {
+        du[a] = U(0, 0, 0) + dt * tu + dt * s_u;
+        dv[a] = V(0, 0, 0) + dt * tv + dt * s_v;
+        dw[a] = W(0, 0, 0) + dt * tw + dt * s_w;
+      } else {
+        du[a] = (U(0, 0, 0) + dt * tu + cur.o[0] + dt * s_u) * T(0.5);
+        dv[a] = (V(0, 0, 0) + dt * tv + cur.o[1] + dt * s_v) * T(0.5);
+        dw[a] = (W(0, 0, 0) + dt * tw + cur.o[2] + dt * s_w) * T(0.5);
+      }
+      nut[((long)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = nu[c];
     }
-    if (nut) nut[((long)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = nu[c];
+    if (kz + 1 < kLz) cur = nxt;
   }
 }
 
