@@ -81,7 +81,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the learned-closure measurement (coarse 96x64x64 grid)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    # about 10 s of the OpenMP oracle on 16 cores at 768x384x384
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of 1 GiB (bandwidth ceiling and the "
                          "scale factor of the FETCH_SIZE/WRITE_SIZE counters) and exit")
