@@ -242,6 +242,34 @@ def test_full_step_parity(name, steps):
         assert div < 1e-12 * umax / dzmin, div
 
 
+def test_full_step_parity_retau550_grid():
+    """The benchmark's whole 768 x 384 x 384 grid (log profile + noise): 10 FP64
+    RK2 steps against the OpenMP oracle, u, v, w and p - <p> to 1e-11 (about
+    20 s of oracle work on 16 cores; 7 host fields of 0.92 GB)."""
+    name, steps = "c3", 10
+    _print_case("full step", name, steps)
+    o = make_oracle(name)
+    u, v, w, u1, v1, w1, p = (o.field() for _ in range(7))
+    o.init_flow(u, v, w, "log", noise_intensity=0.1)
+    o.update_bound_vel(u, v, w)
+    o.update_bound_p(p)
+    with make_channel(name) as ch:
+        for fld, arr in zip(VEL + ("p",), (u, v, w, p)):
+            ch.upload(fld, arr)
+        ch.step(steps)
+        print("  GPU steps done; oracle steps", flush=True)
+        o.step(steps, u, v, w, u1, v1, w1, p)
+        del u1, v1, w1
+        for fld, arr in zip(VEL, (u, v, w)):
+            e = rel_err(interior(ch.download(fld)), interior(arr))
+            print(f"  {fld}: {e:.2e}", flush=True)
+            assert e < 1e-11, (fld, e)
+        a, b = interior(ch.download("p")), interior(p)
+        e = rel_err(a - a.mean(), b - b.mean())
+        print(f"  p - <p>: {e:.2e}", flush=True)
+        assert e < 1e-11
+
+
 @pytest.mark.parametrize("name,precision", [("r3", "fp64"), ("c1", "fp64"), ("r3", "fp32_mixed")])
 def test_multistep_folding_is_bitexact(name, precision):
     """pll_step(n) folds each step's forcing and final boundary update into the
