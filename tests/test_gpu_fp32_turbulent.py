@@ -62,3 +62,52 @@ def test_fp32_statistics_match_fp64_in_turbulence():
     assert dre < 0.02
     for k, v in errs.items():
         assert v < 0.05, (k, v)     # Reynolds stresses, sampling noise of 40 time units
+
+
+# The benchmark's Re_tau~550 grid, whole: 768 x 384 x 384, stretch 1.8,
+# Re_b = 20000.  dt = 0.002 (CFL ~0.35, four times the benchmark's step) keeps
+# the run to about two minutes: 30 time units of spin-up in FP32-mixed, then
+# both paths for 10 time units from that state.
+C3 = dict(nx=768, ny=384, nz=384, lx=2 * math.pi, ly=math.pi, lz=2.0, re=10000.0, dt=0.002)
+C3_SPIN, C3_STEPS, C3_EVERY = 15000, 5000, 20
+
+
+@pytest.mark.timeout(420)
+def test_fp32_mean_profile_matches_fp64_at_retau550():
+    print("CASE fp32 vs fp64 turbulent statistics Re_tau~550 (768x384x384)", flush=True)
+    c = C3
+    dzf = pll.mesh_by_func(1, 0, 1.8, c["nz"], c["lz"])
+    zc = np.cumsum(np.concatenate([[0.5 * dzf[1]], 0.5 * (dzf[1:c["nz"]] + dzf[2:c["nz"] + 1])]))
+    args = (c["nx"], c["ny"], c["nz"], c["lx"], c["ly"], c["lz"], c["re"], c["dt"], dzf)
+    with pll.Channel(*args, precision="fp32_mixed") as ch:
+        ch.initFlow("poi", with_noise=False)
+        cp.seed_turbulence(ch, 0.3, 1, c["lx"], c["ly"], zc)
+        for s in range(0, C3_SPIN, 2500):
+            ch.step(2500)
+            print(f"  spin-up {s + 2500}/{C3_SPIN}", flush=True)
+        state = {f: ch.download(f) for f in ("u", "v", "w", "p")}
+    stats = {}
+    for prec in ("fp64", "fp32_mixed"):
+        with pll.Channel(*args, precision=prec) as ch:
+            for f, a in state.items():
+                ch.upload(f, a)
+            ch.stat_enable((1, 1, 1, 1, 1, 1, 0, 1, 0, 0, 0))
+            n = C3_STEPS // C3_EVERY
+            for s in range(n):
+                ch.step(C3_EVERY)
+                ch.stat_accumulate()
+                if (s + 1) % (n // 5) == 0:
+                    print(f"  {prec}: {(s + 1) * C3_EVERY}/{C3_STEPS} steps", flush=True)
+            assert ch.checkNaN("u")
+            st = cp.plane_stats(ch, n)
+            st["re_tau"] = cp.wall_re_tau(st["U"], dzf[1], c["re"])
+            stats[prec] = st
+    a, b = stats["fp64"], stats["fp32_mixed"]
+    prof = float(np.max(np.abs(b["U"] - a["U"])) / np.max(a["U"]))
+    errs = {k: float(np.max(np.abs(b[k] - a[k])) / np.max(np.abs(a[k]))) for k in ("uu", "vv", "ww", "uw")}
+    dre = abs(b["re_tau"] - a["re_tau"]) / a["re_tau"]
+    print(f"  Re_tau fp64 {a['re_tau']:.1f} fp32 {b['re_tau']:.1f}; mean-profile err {prof:.2e}; "
+          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert 480 < a["re_tau"] < 620  # turbulent near the target friction Reynolds number
+    assert prof < 5e-3              # the 0.5 % mean-profile bar
+    assert dre < 0.02
