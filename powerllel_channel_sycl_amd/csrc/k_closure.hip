@@ -74,7 +74,35 @@ struct Mlp {
   const uint16_t* w1;  // BF16 W1 [16][64]
   const uint16_t* w2;  // BF16 W2 [64][64]
   const uint16_t* w3;  // BF16 W3 [64]
+  // the same BF16 weights as each lane's B operand of the forward products,
+  // 8 values (16 bytes) per lane: b1[t][lane][e] = W1[8h + e][32t + c] and
+  // b2[t][kb][lane][e] = W2[16kb + 8h + e][32t + c], h = lane/32, c = lane%32
+  const uint16_t* b1;
+  const uint16_t* b2;
 };
+
+// Layout of the BF16 weight buffer: W1 | W2 | W3 | b1 | b2 (element offsets).
+constexpr int kWb1 = 0, kWb2 = kWb1 + kInP * kH, kWb3 = kWb2 + kH * kH, kWbB1 = kWb3 + kH,
+              kWbB2 = kWbB1 + 2 * 64 * 8, kWbLen = kWbB2 + 2 * 4 * 64 * 8;
+static_assert(kWbB1 % 8 == 0 && kWbB2 % 8 == 0, "operand copies start on 16-byte boundaries");
+
+// BF16 of parameter i (already computed as v) into the plain and operand copies.
+__device__ inline void store_bf16_weight(uint16_t* __restrict__ wb, int i, float v) {
+  const uint16_t b = to_bf16(v);
+  if (i < oB1) {
+    const int r = (i - oW1) / kH, c = (i - oW1) % kH;
+    wb[kWb1 + i - oW1] = b;
+    const int t = c / 32, lane = 32 * ((r % 16) / 8) + c % 32;
+    wb[kWbB1 + (t * 64 + lane) * 8 + r % 8] = b;
+  } else if (i >= oW2 && i < oB2) {
+    const int r = (i - oW2) / kH, c = (i - oW2) % kH;
+    wb[kWb2 + i - oW2] = b;
+    const int t = c / 32, kb = r / 16, lane = 32 * ((r % 16) / 8) + c % 32;
+    wb[kWbB2 + ((t * 4 + kb) * 64 + lane) * 8 + r % 8] = b;
+  } else if (i >= oW3 && i < oB3) {
+    wb[kWb3 + i - oW3] = b;
+  }
+}
 
 // Per-wave LDS: X (32 x 16), h1 and h2 (32 x 64), deltas (32 x 64), all BF16.
 struct WaveLds {
@@ -120,9 +148,7 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
   const bf16x8 a1 = load8(&L.x[col][kh]);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    bf16x8 b1;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) b1[e] = (short)w.w1[(kh + e) * kH + 32 * t + col];
+    const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(w.b1 + (t * 64 + lane) * 8);
     f32x16 acc = {};
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
     const float bias = w.p[oB1 + 32 * t + col];
@@ -142,9 +168,7 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
 #pragma unroll
     for (int kb = 0; kb < kH / 16; ++kb) {
       const bf16x8 a = load8(&L.h1[col][16 * kb + kh]);
-      bf16x8 b;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) b[e] = (short)w.w2[(16 * kb + kh + e) * kH + 32 * t + col];
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(w.b2 + ((t * 4 + kb) * 64 + lane) * 8);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
     const float bias = w.p[oB2 + 32 * t + col];
@@ -342,9 +366,7 @@ __global__ __launch_bounds__(256) void k_grad_final(const float* __restrict__ ch
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __restrict__ m1,
                                               float* __restrict__ m2, const float* __restrict__ g,
                                               float lr, float b1, float b2, float eps, float c1,
-                                              float c2, uint16_t* __restrict__ w1b,
-                                              uint16_t* __restrict__ w2b,
-                                              uint16_t* __restrict__ w3b) {
+                                              float c2, uint16_t* __restrict__ wb) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= kNParam) return;
   const bool pad = i < oB1 && (i / kH) >= kIn;
@@ -358,18 +380,13 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __re
     v -= lr * (a / c1) / (sqrtf(b / c2) + eps);
     p[i] = v;
   }
-  if (i < oB1) w1b[i - oW1] = to_bf16(pad ? 0.f : v);
-  else if (i >= oW2 && i < oB2) w2b[i - oW2] = to_bf16(v);
-  else if (i >= oW3 && i < oB3) w3b[i - oW3] = to_bf16(v);
+  store_bf16_weight(wb, i, pad ? 0.f : v);
 }
 
-__global__ void k_refresh_bf16(const float* __restrict__ p, uint16_t* __restrict__ w1b,
-                               uint16_t* __restrict__ w2b, uint16_t* __restrict__ w3b) {
+__global__ void k_refresh_bf16(const float* __restrict__ p, uint16_t* __restrict__ wb) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= kNParam) return;
-  if (i < oB1) w1b[i - oW1] = to_bf16(((i / kH) >= kIn) ? 0.f : p[i]);
-  else if (i >= oW2 && i < oB2) w2b[i - oW2] = to_bf16(p[i]);
-  else if (i >= oW3 && i < oB3) w3b[i - oW3] = to_bf16(p[i]);
+  store_bf16_weight(wb, i, (i < oB1 && (i / kH) >= kIn) ? 0.f : p[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -936,7 +953,7 @@ struct pll_closure {
 namespace {
 
 Mlp weights(const pll_closure* c) {
-  return Mlp{c->p, c->wb, c->wb + kInP * kH, c->wb + kInP * kH + kH * kH};
+  return Mlp{c->p, c->wb + kWb1, c->wb + kWb2, c->wb + kWb3, c->wb + kWbB1, c->wb + kWbB2};
 }
 
 int ensure_partial(pll_closure* c, long nwaves) {
@@ -1254,7 +1271,7 @@ int pll_closure_create(uint64_t seed, pll_closure** out) {
   if (hipMalloc(&c->p, pb) != hipSuccess || hipMalloc(&c->m1, pb) != hipSuccess ||
       hipMalloc(&c->m2, pb) != hipSuccess || hipMalloc(&c->grad, pb + sizeof(float)) != hipSuccess ||
       hipMalloc(&c->chunks, (size_t)(kMaxWaves / kRedChunk) * (pb + sizeof(float))) != hipSuccess ||
-      hipMalloc(&c->wb, (size_t)(kInP * kH + kH * kH + kH) * sizeof(uint16_t)) != hipSuccess)
+      hipMalloc(&c->wb, (size_t)kWbLen * sizeof(uint16_t)) != hipSuccess)
     return bail(fail(PLL_ERR_NOMEM, "closure parameters"));
   // Glorot-uniform weights, zero biases (the output bias starts at softplus^-1(1))
   std::vector<float> h(kNParam, 0.f);
@@ -1275,7 +1292,7 @@ int pll_closure_create(uint64_t seed, pll_closure** out) {
       hipMemcpy(c->m2, z.data(), pb, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(PLL_ERR_HIP, "closure initialisation"));
   hipLaunchKernelGGL(k_refresh_bf16, dim3((kNParam + 255) / 256), dim3(256), 0, c->stream, c->p,
-                     c->wb, c->wb + kInP * kH, c->wb + kInP * kH + kH * kH);
+                     c->wb);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(PLL_ERR_HIP, "closure init"));
   *out = c;
   return PLL_OK;
@@ -1307,7 +1324,7 @@ int pll_closure_set_params(pll_closure* c, const float* host) {
   PLL_HIP_TRY(hipStreamSynchronize(c->stream));
   PLL_HIP_TRY(hipMemcpy(c->p, host, (size_t)kNParam * sizeof(float), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_refresh_bf16, dim3((kNParam + 255) / 256), dim3(256), 0, c->stream, c->p,
-                     c->wb, c->wb + kInP * kH, c->wb + kInP * kH + kH * kH);
+                     c->wb);
   PLL_HIP_TRY(hipStreamSynchronize(c->stream));
   return PLL_OK;
 }
@@ -1334,8 +1351,7 @@ int pll_closure_train_step(pll_closure* c, const float* x, const float* y, long 
   const float c1 = (float)(1.0 - std::pow(b1, (double)c->step));
   const float c2 = (float)(1.0 - std::pow(b2, (double)c->step));
   hipLaunchKernelGGL(k_adam, dim3((kNParam + 255) / 256), dim3(256), 0, c->stream, c->p, c->m1,
-                     c->m2, c->grad, (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2, c->wb,
-                     c->wb + kInP * kH, c->wb + kInP * kH + kH * kH);
+                     c->m2, c->grad, (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2, c->wb);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
