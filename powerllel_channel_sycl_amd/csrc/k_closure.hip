@@ -104,6 +104,16 @@ __device__ inline void store_bf16_weight(uint16_t* __restrict__ wb, int i, float
   }
 }
 
+// Orders a wave's LDS writes before its lanes' later LDS reads of them.  A
+// wave's LDS instructions execute in order, so this only has to keep the
+// compiler from moving memory operations across it: the training and predict
+// kernels share no LDS between waves and need no block barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-wave LDS: X (32 x 16), h1 and h2 (32 x 64), deltas (32 x 64), all BF16.
 struct WaveLds {
   uint16_t x[32][kInP];
@@ -138,11 +148,11 @@ __device__ inline void stage_x(const float* __restrict__ x, long m0, long n, L_&
 // Forward pass of one 32-sample tile whose BF16 features are in L.x (written
 // before the call by any lane of the wave): fills L.h1/h2, returns the ReLU
 // activations in D layout and, on lanes 0..31, the output o of sample lane.
-// Every wave of the block calls it the same number of times (it holds barriers).
+// Only the calling wave's LDS is used.
 template <class L_>
 __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2], f32x16 h2[2],
                                     float& out_o) {
-  __syncthreads();
+  wave_lds_sync();
   const int col = lane % 32, kh = 8 * (lane / 32);
   // layer 1: h1 = relu(X W1 + b1)
   const bf16x8 a1 = load8(&L.x[col][kh]);
@@ -160,7 +170,7 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
     }
     h1[t] = acc;
   }
-  __syncthreads();
+  wave_lds_sync();
   // layer 2: h2 = relu(h1 W2 + b2)
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -180,7 +190,7 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
     }
     h2[t] = acc;
   }
-  __syncthreads();
+  wave_lds_sync();
   // output: o = h2 . W3 + b3 (lanes 0..31, one sample each, fixed order)
   out_o = 0.f;
   if (lane < 32) {
@@ -210,14 +220,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
   const long nwaves = (long)gridDim.x * kWaves;
   for (int tile = 0; tile < tiles_per_wave; ++tile) {
     const long m0 = (gw + tile * nwaves) * 32;
-    // every wave of a block runs the same number of tiles (barriers inside)
     f32x16 h1[2], h2[2];
     float o;
     stage_x(x, m0, n, L, lane);
     forward_tile(w, L, lane, h1, h2, o);
     if (!TRAIN) {
       if (lane < 32 && m0 + lane < n) out[m0 + lane] = softplus(o);
-      __syncthreads();
+      wave_lds_sync();
       continue;
     }
     // dL/do
@@ -231,7 +240,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
       }
       L.g[lane] = g;
     }
-    __syncthreads();
+    wave_lds_sync();
     // output layer gradients and delta2 = g w3 relu'(z2)
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -250,7 +259,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
     }
     if (lane == 0)
       for (int m = 0; m < 32; ++m) gb3 += L.g[m];
-    __syncthreads();
+    wave_lds_sync();
     // dW2 += h1^T d2 : rows k (h1 column), cols n, K = samples
 #pragma unroll
     for (int tk = 0; tk < 2; ++tk)
@@ -284,7 +293,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
         L.d1[drow(j, lane)][32 * t + col] = to_bf16(d1);
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     // dW1 += X^T d1 : rows i (< 16), cols k, K = samples
 #pragma unroll
     for (int tn = 0; tn < 2; ++tn)
@@ -298,7 +307,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
         }
         gW1[tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, gW1[tn], 0, 0, 0);
       }
-    __syncthreads();
+    wave_lds_sync();
   }
   if (!TRAIN) return;
   // write this wave's partial: lanes l and l+32 hold different rows of the same
@@ -320,9 +329,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
     }
   auto colsum = [&](float v, int t, int off) {
     L.red[lane] = v;
-    __syncthreads();
+    wave_lds_sync();
     if (lane < 32) P[off + 32 * t + lane] = L.red[lane] + L.red[lane + 32];
-    __syncthreads();
+    wave_lds_sync();
   };
   for (int t = 0; t < 2; ++t) {
     colsum(gb1[t], t, oB1);
@@ -330,7 +339,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
     colsum(gw3[t], t, oW3);
   }
   L.red[lane] = lane < 32 ? loss : 0.f;
-  __syncthreads();
+  wave_lds_sync();
   if (lane == 0) {
     float l = 0.f;
     for (int q = 0; q < 32; ++q) l += L.red[q];
