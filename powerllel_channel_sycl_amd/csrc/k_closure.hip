@@ -114,13 +114,12 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-wave LDS: X (32 x 16), h1 and h2 (32 x 64), deltas (32 x 64), all BF16.
+// Per-wave LDS: X (32 x 16), h1, h2 and delta2 (32 x 64), all BF16.
 struct WaveLds {
   uint16_t x[32][kInP];
   uint16_t h1[32][kH];
   uint16_t h2[32][kH];
   uint16_t d2[32][kH];
-  uint16_t d1[32][kH];
   float g[32];    // dL/do per sample
   float red[64];  // lane exchange
 };
@@ -130,6 +129,14 @@ __device__ inline bf16x8 load8(const uint16_t* p) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) v[e] = (short)p[e];
   return v;
+}
+
+// BF16 (nearest even) of accumulator elements j0 .. j0+7 as one MFMA operand.
+__device__ inline bf16x8 pack8_bf16(const f32x16& v, int j0) {
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) r[e] = (short)to_bf16(v[j0 + e]);
+  return r;
 }
 
 // D-layout row of accumulator element j for this lane.
@@ -241,7 +248,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
       L.g[lane] = g;
     }
     wave_lds_sync();
-    // output layer gradients and delta2 = g w3 relu'(z2)
+    // output layer gradients and delta2 = g w3 relu'(z2), in the D layout
+    // (the lane's column nn, samples drow(j, lane))
+    f32x16 d2r[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int nn = 32 * t + col;
@@ -250,32 +259,36 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
       for (int j = 0; j < 16; ++j) {
         const int m = drow(j, lane);
         const float g = L.g[m];
-        gw3[t] += from_bf16(L.h2[m][nn]) * g;
         const float h = h2[t][j];
+        gw3[t] += from_bf16(to_bf16(h)) * g;
         const float d2 = h > 0.f ? g * w3n : 0.f;
         gb2[t] += d2;
+        d2r[t][j] = d2;
         L.d2[m][nn] = to_bf16(d2);
       }
     }
     if (lane == 0)
       for (int m = 0; m < 32; ++m) gb3 += L.g[m];
     wave_lds_sync();
-    // dW2 += h1^T d2 : rows k (h1 column), cols n, K = samples
+    // dW2 += h1^T d2 : rows k (h1 column), cols n, K = samples.  Both operands
+    // come from the D-layout registers: K-slot e of the lane's half in block ks
+    // is sample drow(8 ks + e, lane), the same in A and B.
 #pragma unroll
-    for (int tk = 0; tk < 2; ++tk)
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a[2], b[2];
 #pragma unroll
-      for (int tn = 0; tn < 2; ++tn)
+      for (int t = 0; t < 2; ++t) {
+        a[t] = pack8_bf16(h1[t], 8 * ks);
+        b[t] = pack8_bf16(d2r[t], 8 * ks);
+      }
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-          bf16x8 a, b;
+      for (int tk = 0; tk < 2; ++tk)
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            a[e] = (short)L.h1[16 * ks + kh + e][32 * tk + col];
-            b[e] = (short)L.d2[16 * ks + kh + e][32 * tn + col];
-          }
-          gW2[tk][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, gW2[tk][tn], 0, 0, 0);
-        }
+        for (int tn = 0; tn < 2; ++tn)
+          gW2[tk][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[tk], b[tn], gW2[tk][tn], 0, 0, 0);
+    }
     // delta1 = (d2 W2^T) relu'(z1) : rows m, cols k, K = n
+    f32x16 d1r[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       f32x16 acc = {};
@@ -290,23 +303,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
         const float h = h1[t][j];
         const float d1 = h > 0.f ? acc[j] : 0.f;
         gb1[t] += d1;
-        L.d1[drow(j, lane)][32 * t + col] = to_bf16(d1);
+        acc[j] = d1;
       }
+      d1r[t] = acc;
     }
-    wave_lds_sync();
-    // dW1 += X^T d1 : rows i (< 16), cols k, K = samples
+    // dW1 += X^T d1 : rows i (< 16), cols k, K = samples in the order of dW2
 #pragma unroll
-    for (int tn = 0; tn < 2; ++tn)
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a;
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        bf16x8 a, b;
+      for (int e = 0; e < 8; ++e)
+        a[e] = col < kInP ? (short)L.x[drow(8 * ks + e, lane)][col] : (short)0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          a[e] = col < kInP ? (short)L.x[16 * ks + kh + e][col] : (short)0;
-          b[e] = (short)L.d1[16 * ks + kh + e][32 * tn + col];
-        }
-        gW1[tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, gW1[tn], 0, 0, 0);
-      }
+      for (int tn = 0; tn < 2; ++tn)
+        gW1[tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, pack8_bf16(d1r[tn], 8 * ks), gW1[tn],
+                                                          0, 0, 0);
+    }
     wave_lds_sync();
   }
   if (!TRAIN) return;
