@@ -120,6 +120,7 @@ struct WaveLds {
   uint16_t h1[32][kH];
   uint16_t h2[32][kH];
   uint16_t d2[32][kH];
+  float o[32];    // network output per sample
   float g[32];    // dL/do per sample
   float red[64];  // lane exchange
 };
@@ -198,13 +199,20 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
     h2[t] = acc;
   }
   wave_lds_sync();
-  // output: o = h2 . W3 + b3 (lanes 0..31, one sample each, fixed order)
-  out_o = 0.f;
-  if (lane < 32) {
-    float o = 0.f;
-    for (int k = 0; k < kH; ++k) o += from_bf16(L.h2[lane][k]) * from_bf16(w.w3[k]);
-    out_o = o + w.p[oB3];
+  // output: o = h2 . W3 + b3 as one more product, B holding W3 in all 32
+  // columns (K = hidden units); column 0 goes back to lanes 0..31 through LDS
+  f32x16 acc = {};
+#pragma unroll
+  for (int kb = 0; kb < kH / 16; ++kb) {
+    const bf16x8 a = load8(&L.h2[col][16 * kb + kh]);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(w.w3 + 16 * kb + kh);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
+  if (col == 0)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) L.o[drow(j, lane)] = acc[j];
+  wave_lds_sync();
+  out_o = lane < 32 ? L.o[lane] + w.p[oB3] : 0.f;
 }
 
 // Training: one partial gradient (and loss sum) per wave.  Loss = mean over
