@@ -14,8 +14,9 @@
 // back-propagated deltas -- so a CPU restatement can follow it exactly.
 //
 // Training is deterministic: each wave accumulates its gradient over its
-// tiles in registers, writes one partial per wave, and a second kernel sums
-// the partials in a fixed order before the Adam update.
+// tiles in registers, a block sums its waves' partials in LDS in wave order
+// and writes one partial, and a second kernel sums the partials in a fixed
+// order before the Adam update.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -38,8 +39,8 @@ constexpr int kH = 64;    // hidden width
 constexpr int oW1 = 0, oB1 = oW1 + kInP * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH,
               oW3 = oB2 + kH, oB3 = oW3 + kH, kNParam = oB3 + 1;
 constexpr int kWaves = 4;          // waves per block
-constexpr long kMaxWaves = 1024;   // training grid: one wave per SIMD (partials kMaxWaves x kNParam)
-constexpr int kRedChunk = 32;      // wave partials summed per chunk in the first reduction level
+constexpr long kMaxWaves = 1024;   // training grid: one wave per SIMD (one partial per kWaves waves)
+constexpr int kRedChunk = 32;      // block partials summed per chunk in the first reduction level
 
 typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
@@ -215,7 +216,7 @@ __device__ inline void forward_tile(const Mlp& w, L_& L, int lane, f32x16 h1[2],
   out_o = lane < 32 ? L.o[lane] + w.p[oB3] : 0.f;
 }
 
-// Training: one partial gradient (and loss sum) per wave.  Loss = mean over
+// Training: one partial gradient (and loss sum) per block.  Loss = mean over
 // the n samples of (softplus(o) - y)^2.
 template <bool TRAIN>
 __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restrict__ x,
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
                                                       float* __restrict__ partial,
                                                       int tiles_per_wave) {
   __shared__ WaveLds lds[kWaves];
+  __shared__ float bp[TRAIN ? kWaves : 1][kNParam + 1];  // the waves' partial gradients
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   WaveLds& L = lds[wave];
   const long gw = (long)blockIdx.x * kWaves + wave;  // global wave index
@@ -330,9 +332,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
     wave_lds_sync();
   }
   if (!TRAIN) return;
-  // write this wave's partial: lanes l and l+32 hold different rows of the same
-  // columns; the per-column sums (biases, W3) are combined through LDS
-  float* P = partial + gw * (kNParam + 1);
+  // this wave's partial into LDS: lanes l and l+32 hold different rows of the
+  // same columns; the per-column sums (biases, W3) are combined through LDS
+  float* P = bp[wave];
 #pragma unroll
   for (int tk = 0; tk < 2; ++tk)
 #pragma unroll
@@ -366,17 +368,26 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp(Mlp w, const float* __restr
     P[oB3] = gb3;
     P[kNParam] = l;
   }
+  // the block's partial: its waves' partials summed in wave order
+  __syncthreads();
+  float* Q = partial + (long)blockIdx.x * (kNParam + 1);
+  for (int i = threadIdx.x; i <= kNParam; i += 64 * kWaves) {
+    float v = bp[0][i];
+#pragma unroll
+    for (int q = 1; q < kWaves; ++q) v += bp[q][i];
+    Q[i] = v;
+  }
 }
 
-// Sum of the per-wave partials in a fixed order, in two levels: chunk c of
-// kRedChunk waves into chunks[c], then the chunks in order into grad
+// Sum of the per-block partials in a fixed order, in two levels: chunk c of
+// kRedChunk partials into chunks[c], then the chunks in order into grad
 // (grad[i] for i < kNParam, the loss sum last).
 __global__ __launch_bounds__(256) void k_grad_reduce(const float* __restrict__ partial,
-                                                     long nwaves, float* __restrict__ chunks) {
+                                                     long nparts, float* __restrict__ chunks) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const long c = blockIdx.y;
   if (i > kNParam) return;
-  const long q1 = std::min(nwaves, (c + 1) * kRedChunk);
+  const long q1 = std::min(nparts, (c + 1) * kRedChunk);
   float s = 0.f;
   for (long q = c * kRedChunk; q < q1; ++q) s += partial[q * (kNParam + 1) + i];
   chunks[c * (kNParam + 1) + i] = s;
@@ -969,7 +980,7 @@ struct pll_closure {
   float* m2 = nullptr;
   uint16_t* wb = nullptr;  // BF16 W1 | W2 | W3
   float* partial = nullptr;
-  long npartial_waves = 0;
+  long npartial_blocks = 0;  // partial gradients, one per k_mlp block
   float* grad = nullptr;    // kNParam + 1 (loss sum last)
   float* chunks = nullptr;  // first-level gradient sums, (kMaxWaves / kRedChunk) x (kNParam + 1)
   long step = 0;
@@ -985,12 +996,12 @@ Mlp weights(const pll_closure* c) {
   return Mlp{c->p, c->wb + kWb1, c->wb + kWb2, c->wb + kWb3, c->wb + kWbB1, c->wb + kWbB2};
 }
 
-int ensure_partial(pll_closure* c, long nwaves) {
-  if (nwaves <= c->npartial_waves) return PLL_OK;
+int ensure_partial(pll_closure* c, long nblocks) {
+  if (nblocks <= c->npartial_blocks) return PLL_OK;
   if (c->partial) (void)hipFree(c->partial);
   c->partial = nullptr;
-  PLL_HIP_TRY(hipMalloc(&c->partial, (size_t)nwaves * (kNParam + 1) * sizeof(float)));
-  c->npartial_waves = nwaves;
+  PLL_HIP_TRY(hipMalloc(&c->partial, (size_t)nblocks * (kNParam + 1) * sizeof(float)));
+  c->npartial_blocks = nblocks;
   return PLL_OK;
 }
 
@@ -1009,14 +1020,15 @@ int tiles_per_wave(long n, long nwaves) {
 
 int gradient(pll_closure* c, const float* x, const float* y, long n) {
   const long nw = waves_for(n);
-  int rc = ensure_partial(c, nw);
+  int rc = ensure_partial(c, nw / kWaves);
   if (rc) return rc;
   hipLaunchKernelGGL((k_mlp<true>), dim3((unsigned)(nw / kWaves)), dim3(64 * kWaves), 0, c->stream,
                      weights(c), x, y, n, static_cast<float*>(nullptr), c->partial,
                      tiles_per_wave(n, nw));
-  const long nchunk = (nw + kRedChunk - 1) / kRedChunk;
+  const long nblk = nw / kWaves;  // one partial per block
+  const long nchunk = (nblk + kRedChunk - 1) / kRedChunk;
   hipLaunchKernelGGL(k_grad_reduce, dim3((kNParam + 1 + 255) / 256, (unsigned)nchunk), dim3(256), 0,
-                     c->stream, c->partial, nw, c->chunks);
+                     c->stream, c->partial, nblk, c->chunks);
   hipLaunchKernelGGL(k_grad_final, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream,
                      c->chunks, nchunk, c->grad);
   PLL_HIP_TRY(hipGetLastError());
