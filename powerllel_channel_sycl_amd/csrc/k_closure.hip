@@ -402,17 +402,25 @@ __global__ __launch_bounds__(256) void k_grad_final(const float* __restrict__ ch
 }
 
 // Adam (Kingma & Ba, 2015) on the FP32 master parameters; refreshes the BF16
-// weights.  The padded rows 10..15 of W1 stay zero.
+// weights.  The padded rows 10..15 of W1 stay zero.  The gradient is the
+// chunk sums of k_grad_reduce added in k_grad_final's order (so the same
+// bits), also stored in grad with the loss sum.
 __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __restrict__ m1,
-                                              float* __restrict__ m2, const float* __restrict__ g,
-                                              float lr, float b1, float b2, float eps, float c1,
-                                              float c2, uint16_t* __restrict__ wb) {
+                                              float* __restrict__ m2,
+                                              const float* __restrict__ chunks, long nchunk,
+                                              float* __restrict__ grad, float lr, float b1,
+                                              float b2, float eps, float c1, float c2,
+                                              uint16_t* __restrict__ wb) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= kNParam) return;
+  if (i > kNParam) return;
+  float gs = 0.f;
+  for (long c = 0; c < nchunk; ++c) gs += chunks[c * (kNParam + 1) + i];
+  grad[i] = gs;
+  if (i == kNParam) return;  // the loss sum
   const bool pad = i < oB1 && (i / kH) >= kIn;
   float v = p[i];
   if (!pad) {
-    const float gi = g[i];
+    const float gi = gs;
     const float a = b1 * m1[i] + (1.f - b1) * gi;
     const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
     m1[i] = a;
@@ -1018,10 +1026,12 @@ int tiles_per_wave(long n, long nwaves) {
   return (int)((tiles + nwaves - 1) / nwaves);
 }
 
-int gradient(pll_closure* c, const float* x, const float* y, long n) {
+// The gradient into chunk sums; with `final` also summed into c->grad (the
+// training step leaves the last level to k_adam).  Returns the chunk count.
+long gradient_chunks(pll_closure* c, const float* x, const float* y, long n, bool final, int* rc) {
   const long nw = waves_for(n);
-  int rc = ensure_partial(c, nw / kWaves);
-  if (rc) return rc;
+  *rc = ensure_partial(c, nw / kWaves);
+  if (*rc) return 0;
   hipLaunchKernelGGL((k_mlp<true>), dim3((unsigned)(nw / kWaves)), dim3(64 * kWaves), 0, c->stream,
                      weights(c), x, y, n, static_cast<float*>(nullptr), c->partial,
                      tiles_per_wave(n, nw));
@@ -1029,10 +1039,18 @@ int gradient(pll_closure* c, const float* x, const float* y, long n) {
   const long nchunk = (nblk + kRedChunk - 1) / kRedChunk;
   hipLaunchKernelGGL(k_grad_reduce, dim3((kNParam + 1 + 255) / 256, (unsigned)nchunk), dim3(256), 0,
                      c->stream, c->partial, nblk, c->chunks);
-  hipLaunchKernelGGL(k_grad_final, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream,
-                     c->chunks, nchunk, c->grad);
-  PLL_HIP_TRY(hipGetLastError());
-  return PLL_OK;
+  if (final)
+    hipLaunchKernelGGL(k_grad_final, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream,
+                       c->chunks, nchunk, c->grad);
+  const hipError_t e = hipGetLastError();
+  *rc = e == hipSuccess ? PLL_OK : hip_fail(e, "closure gradient kernels");
+  return nchunk;
+}
+
+int gradient(pll_closure* c, const float* x, const float* y, long n) {
+  int rc = PLL_OK;
+  gradient_chunks(c, x, y, n, true, &rc);
+  return rc;
 }
 
 // Label regularisation: eps of nu* in units of the squared scaled strain rate,
@@ -1385,14 +1403,16 @@ int pll_closure_gradient(pll_closure* c, const float* x, const float* y, long n,
 
 int pll_closure_train_step(pll_closure* c, const float* x, const float* y, long n, double lr) {
   if (!c || !x || !y || n <= 0 || !(lr > 0)) return fail(PLL_ERR_ARG, "bad arguments");
-  int rc = gradient(c, x, y, n);
+  int rc = PLL_OK;
+  const long nchunk = gradient_chunks(c, x, y, n, false, &rc);
   if (rc) return rc;
   c->step += 1;
   const double b1 = 0.9, b2 = 0.999;
   const float c1 = (float)(1.0 - std::pow(b1, (double)c->step));
   const float c2 = (float)(1.0 - std::pow(b2, (double)c->step));
-  hipLaunchKernelGGL(k_adam, dim3((kNParam + 255) / 256), dim3(256), 0, c->stream, c->p, c->m1,
-                     c->m2, c->grad, (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2, c->wb);
+  hipLaunchKernelGGL(k_adam, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream, c->p,
+                     c->m1, c->m2, c->chunks, nchunk, c->grad, (float)lr, (float)b1, (float)b2,
+                     1e-8f, c1, c2, c->wb);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
