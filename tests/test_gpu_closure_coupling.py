@@ -104,7 +104,8 @@ def _closure_with_params(seed, re):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
-@pytest.mark.parametrize("name,crf", [("r235", 0.0), ("gen", 0.0), ("ragged", 0.0), ("r235", 0.3)])
+@pytest.mark.parametrize("name,crf", [("r235", 0.0), ("gen", 0.0), ("ragged", 0.0), ("tiny", 0.0),
+                                      ("r235", 0.3)])
 def test_closure_stages_match_oracle(name, crf, precision):
     print(f"CASE closure stages {name} u_crf={crf} {precision}", flush=True)
     prm = params(name)
