@@ -31,6 +31,9 @@ WORKLOADS = {
                              2800.0, 1e-3, "poi"),
     "retau180_64x64x64": (64, 64, 64, 6.283185307179586, 3.141592653589793, 2.0, 1.5, 2800.0,
                           2e-3, "poi"),
+    # the single-GPU memory-sizing case: 604 M points, ~60 GB of FP64 fields
+    "retau1000_1536x768x512": (1536, 768, 512, 6.283185307179586, 3.141592653589793, 2.0, 2.0,
+                               20000.0, 2.5e-4, "log"),
 }
 DEFAULT_WORKLOAD = "retau550_768x384x384"
 

@@ -25,6 +25,8 @@ CASES = {
     "tall": (16, 12, 300, 2 * math.pi, math.pi, 2.0, 1.8, 2800.0, 2e-3),
     # the Re_tau~550 grid of the benchmark, whole
     "c3": (768, 384, 384, 2 * math.pi, math.pi, 2.0, 1.8, 10000.0, 5e-4),
+    # the Re_tau~1000 grid, whole (604 M points)
+    "c4": (1536, 768, 512, 2 * math.pi, math.pi, 2.0, 2.0, 20000.0, 2.5e-4),
     # no dimension a multiple of the closure stage's 32 x 8 x 4 blocks
     "ragged": (40, 30, 10, 2 * math.pi, math.pi, 2.0, 1.5, 2800.0, 2e-3),
 }

@@ -99,12 +99,14 @@ def test_fp32_short_horizon_pointwise(name, steps):
         assert abs(ch.Mean("u") - 1.0) < 1e-6
 
 
-def test_fp32_short_horizon_retau550_grid():
-    """The benchmark's whole 768 x 384 x 384 grid (log profile + noise): 10
+@pytest.mark.parametrize("name", ["c3", "c4"])
+def test_fp32_short_horizon_full_grids(name):
+    """The whole 768 x 384 x 384 and 1536 x 768 x 512 grids (log profile +
+    noise; the second needs ~90 GB of device memory for the two contexts): 10
     FP32-mixed steps against 10 FP64 steps of the GPU path (which matches the
-    oracle on this grid to ~5e-15, test_gpu_parity), u, v, w within 1e-5 of
-    max|u|, divergence at the FP32 floor, bulk velocity held."""
-    name, steps = "c3", 10
+    oracle on the first grid to ~5e-15, test_gpu_parity), u, v, w within 1e-5
+    of max|u|, divergence at the FP32 floor, bulk velocity held."""
+    steps = 10
     print("CASE fp32 short", name, steps, flush=True)
     with make_channel(name) as ref, make_channel(name, precision="fp32_mixed") as ch:
         for c in (ref, ch):
