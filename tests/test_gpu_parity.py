@@ -242,11 +242,12 @@ def test_full_step_parity(name, steps):
         assert div < 1e-12 * umax / dzmin, div
 
 
-def test_full_step_parity_retau550_grid():
-    """The benchmark's whole 768 x 384 x 384 grid (log profile + noise): 10 FP64
-    RK2 steps against the OpenMP oracle, u, v, w and p - <p> to 1e-11 (about
-    20 s of oracle work on 16 cores; 7 host fields of 0.92 GB)."""
-    name, steps = "c3", 10
+@pytest.mark.parametrize("name,steps", [("c3", 10), ("c4", 2)])
+def test_full_step_parity_full_grids(name, steps):
+    """The whole 768 x 384 x 384 and 1536 x 768 x 512 grids (log profile +
+    noise): FP64 RK2 steps against the OpenMP oracle, u, v, w and p - <p> to
+    1e-11 (about 20 s of oracle work on 16 cores for each grid; 7 host fields of
+    0.92 GB and 4.9 GB)."""
     _print_case("full step", name, steps)
     o = make_oracle(name)
     u, v, w, u1, v1, w1, p = (o.field() for _ in range(7))
