@@ -7,16 +7,20 @@
 // v_mfma_f32_32x32x16_bf16 products: A is 32 x 16 (lane l holds row l%32,
 // k = 8*(l/32) .. +7), B is 16 x 32 (lane l holds column l%32, the same k),
 // D is 32 x 32 FP32 (lane l holds column l%32, rows 8*(j/4) + 4*(l/32) + j%4
-// for j = 0..15).  Activations pass between layers through LDS as BF16 (the
-// A operand of the next product); the output layer (one column) is a
-// per-sample FP32 dot product.  Rounding to BF16 is round-to-nearest-even at
-// fixed points -- the inputs, the weights, the activations h1 and h2 and the
-// back-propagated deltas -- so a CPU restatement can follow it exactly.
+// for j = 0..15).  In training and batch inference activations pass between
+// layers through LDS as BF16 (the A operand of the next product), the output
+// layer is one more product with W3 in every B column, and the weight
+// gradients take both operands from the accumulator registers.  The closure
+// stage computes the layers transposed, entirely in registers, with the
+// output layer as BF16 dot products.  Rounding to BF16 is round-to-nearest-
+// even at fixed points -- the inputs, the weights, the activations h1 and h2
+// and the back-propagated deltas -- so a CPU restatement can follow it
+// exactly; only the FP32 summation order differs.
 //
 // Training is deterministic: each wave accumulates its gradient over its
 // tiles in registers, a block sums its waves' partials in LDS in wave order
-// and writes one partial, and a second kernel sums the partials in a fixed
-// order before the Adam update.
+// and writes one partial, and two more levels sum the partials in a fixed
+// order, the last inside the Adam update.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
