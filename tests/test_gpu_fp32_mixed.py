@@ -77,7 +77,7 @@ def test_fp32_trid_matches_oracle(name):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("name,steps", [("c1", 10), ("r3", 10)])
+@pytest.mark.parametrize("name,steps", [("c1", 10), ("r3", 10), ("c2", 10), ("ragged", 10)])
 def test_fp32_short_horizon_pointwise(name, steps):
     """10 steps: u, v, w within 1e-5 (relative to max|u|) of the FP64 oracle,
     divergence at the FP32 floor, bulk velocity held to 1e-6."""
