@@ -89,6 +89,9 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of 1 GiB (bandwidth ceiling and the "
                          "scale factor of the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--stat-probe", action="store_true",
+                    help="time the run-time statistics accumulation (all 11 sums) on the "
+                         "workload grid in both precisions and exit")
     return ap.parse_args()
 
 
@@ -166,6 +169,35 @@ def copy_probe():
     gbs = 2 * n * 8 / sec / 1e9
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
                       "copy_GBps": round(gbs, 1), "copy_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}))
+
+
+# Statistics accumulation per point: u, v, w, p read once (the neighbours
+# u(i-1), v(j-1) and w(k-1) come from cache), each of the 11 FP64 sums read
+# and written.
+STAT_BYTES = {"fp64": 4 * 8 + 11 * 16, "fp32_mixed": 4 * 4 + 11 * 16}
+
+
+def stat_probe(wl, reps=10):
+    """Wall-clock timing of pll_stat_accumulate (one launch each) on the
+    workload grid, after two untimed calls, in both precisions."""
+    out = {"workload": wl}
+    for precision in ("fp64", "fp32_mixed"):
+        ch, npts = build_channel(wl, precision)
+        with ch:
+            ch.stat_enable((1,) * 11)
+            for _ in range(2):
+                ch.stat_accumulate()
+            ch.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ch.stat_accumulate()
+            ch.synchronize()
+            us = (time.perf_counter() - t0) / reps * 1e6
+            gbs = STAT_BYTES[precision] * npts / (us * 1e-6) / 1e9
+            out[precision] = {"us_per_accumulate": round(us, 1),
+                              "bytes_per_point": STAT_BYTES[precision],
+                              "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    print(json.dumps(out))
 
 
 def build_channel(wl, precision):
@@ -263,6 +295,9 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.stat_probe:
+        stat_probe(args.workload)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
