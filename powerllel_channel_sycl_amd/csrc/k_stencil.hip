@@ -104,6 +104,18 @@ __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W
 constexpr int kRkTx = 64;  // tile width along x (one wavefront per row)
 constexpr int kRkTy = 8;   // tile rows along y
 
+// Tile rows of the stage and projection kernels: 4 for the FP64 stage 1 and
+// projection (4-7 % faster than 8 at 768x384x384 on MI355X), 8 elsewhere (the
+// FP32 stages and the fused FP64 stage 2 lose 10-28 % with 4).
+template <typename T, int STAGE>
+constexpr int rk_ty() {
+  return std::is_same<T, double>::value && STAGE == 1 ? 4 : kRkTy;
+}
+template <typename T>
+constexpr int correct_ty() {
+  return std::is_same<T, double>::value ? 4 : kRkTy;
+}
+
 // Stage 1: d = s + dt*F(s).  Stage 2: d = (s + dt*F(s) + d)/2.
 // A block owns a 64x8 (i,j) tile and marches a chunk of k planes.  The three
 // source fields of planes k-1, k, k+1 (tile plus a one-cell halo) sit in an
@@ -128,7 +140,7 @@ constexpr int kRkTy = 8;   // tile rows along y
 // bulk mean of the next projected velocity, since the projection changes every
 // x-row's sum only by p(nx+1) - p(1) = 0.
 template <typename T, int STAGE>
-__global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict__ su,
+__global__ __launch_bounds__((kRkTx * rk_ty<T, STAGE>())) void k_rk(Geom g, const T* __restrict__ su,
                                                      const T* __restrict__ sv,
                                                      const T* __restrict__ sw, T* __restrict__ du,
                                                      T* __restrict__ dv, T* __restrict__ dw,
@@ -143,7 +155,8 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_rk(Geom g, const T* __restrict
                                                      T dtdxi, T dtdyi, int acc,
                                                      const double* __restrict__ dzflzi,
                                                      double* __restrict__ partial, int bcload) {
-  constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY, NT = TX * TY;
+  constexpr int TX = kRkTx, TY = rk_ty<T, STAGE>(), SX = TX + 2, SY = TY + 2, PL = SX * SY,
+                NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
   // the deferred projection and the predictor's mean serve the FP32-mixed step
   // only; compiling them out of FP64 keeps its register count and occupancy
@@ -571,7 +584,7 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_calc_rhs(Geom g, const T* __re
 // With ACC the block also sums the corrected u times dzflzi(k) (FP64) into
 // partial[block] -- the first half of the bulk mean that forceVel needs.
 template <typename T, bool ACC>
-__global__ __launch_bounds__(kRkTx* kRkTy) void k_correct(Geom g, const T* __restrict__ p,
+__global__ __launch_bounds__(kRkTx* correct_ty<T>()) void k_correct(Geom g, const T* __restrict__ p,
                                                           T* __restrict__ u, T* __restrict__ v,
                                                           T* __restrict__ w,
                                                           const T* __restrict__ dzc_inv, T dtdxi,
@@ -581,8 +594,9 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_correct(Geom g, const T* __res
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
   const int kc = (int)(b / (unsigned)(ntx * nty));
+  constexpr int TY = correct_ty<T>();
   const int i = ti * kRkTx + (int)(threadIdx.x % kRkTx) + 1;
-  const int j = tj * kRkTy + (int)(threadIdx.x / kRkTx) + 1;
+  const int j = tj * TY + (int)(threadIdx.x / kRkTx) + 1;
   const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
   double acc = 0.0;
   if (i <= g.nx && j <= g.ny && k0 <= k1) {
@@ -599,10 +613,10 @@ __global__ __launch_bounds__(kRkTx* kRkTy) void k_correct(Geom g, const T* __res
     }
   }
   if (ACC) {
-    __shared__ double sh[kRkTx * kRkTy];
+    __shared__ double sh[kRkTx * TY];
     sh[threadIdx.x] = acc;
     __syncthreads();
-    for (int h = kRkTx * kRkTy / 2; h > 0; h >>= 1) {
+    for (int h = kRkTx * TY / 2; h > 0; h >>= 1) {
       if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
       __syncthreads();
     }
@@ -976,10 +990,10 @@ struct TileGrid {
   unsigned blocks() const { return (unsigned)ntx * nty * nkc; }
 };
 
-inline TileGrid tile_grid(const pll_context* c) {
+inline TileGrid tile_grid(const pll_context* c, int ty = kRkTy) {
   TileGrid t;
   t.ntx = (c->g.nx + kRkTx - 1) / kRkTx;
-  t.nty = (c->g.ny + kRkTy - 1) / kRkTy;
+  t.nty = (c->g.ny + ty - 1) / ty;
   const long cols = (long)t.ntx * t.nty;
   long want = (2048 + cols - 1) / cols;
   int kc = (int)((c->g.nz + want - 1) / want);
@@ -1038,7 +1052,8 @@ int rk_coef_setup(pll_context* c) {
 
 template <typename T>
 int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
-  const TileGrid t = tile_grid(c);
+  const int ty = stage == 1 ? rk_ty<T, 1>() : rk_ty<T, 2>();
+  const TileGrid t = tile_grid(c, ty);
   if (acc && (long)t.blocks() > c->npartial)
     return fail(PLL_ERR_STATE, "reduction scratch smaller than the stencil grid");
   // the projected loads read p at the periodic image of halo cells, which only
@@ -1054,7 +1069,7 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
   const int crf = c->u_crf != 0.0;
   if (stage == 1) {
     TimerScope ts(c, K_RK1);
-    hipLaunchKernelGGL((k_rk<T, 1>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
+    hipLaunchKernelGGL((k_rk<T, 1>), dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
                        F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
                        F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
@@ -1062,7 +1077,7 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
                        dtdyi, 0, dzflzi, c->d_partial, fold);
   } else {
     TimerScope ts(c, K_RK2);
-    hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream, c->g,
+    hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
                        F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
                        F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, 0, fold,
@@ -1124,7 +1139,8 @@ int rhs_t(pll_context* c) {
 
 template <typename T>
 int correct_t(pll_context* c, bool with_mean) {
-  const TileGrid t = tile_grid(c);
+  constexpr int ty = correct_ty<T>();
+  const TileGrid t = tile_grid(c, ty);
   const double dtdxi = c->dt * c->dx_inv;  // src/mod_calcVel.cc:256-258
   const double dtdyi = c->dt * c->dy_inv;
   const double* dzflzi = c->d_mesh64 + 3 * (c->g.nz + 2);
@@ -1133,12 +1149,12 @@ int correct_t(pll_context* c, bool with_mean) {
   {
     TimerScope ts(c, K_CORRECT);
     if (with_mean)
-      hipLaunchKernelGGL((k_correct<T, true>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0, c->stream,
+      hipLaunchKernelGGL((k_correct<T, true>), dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream,
                          c->g, F<T>(c, PLL_P), F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W),
                          mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt, t.kchunk, t.ntx,
                          t.nty, dzflzi, c->d_partial);
     else
-      hipLaunchKernelGGL((k_correct<T, false>), dim3(t.blocks()), dim3(kRkTx * kRkTy), 0,
+      hipLaunchKernelGGL((k_correct<T, false>), dim3(t.blocks()), dim3(kRkTx * ty), 0,
                          c->stream, c->g, F<T>(c, PLL_P), F<T>(c, PLL_U), F<T>(c, PLL_V),
                          F<T>(c, PLL_W), mesh_of<T>(c).dzc_inv, (T)dtdxi, (T)dtdyi, (T)c->dt,
                          t.kchunk, t.ntx, t.nty, dzflzi, c->d_partial);
