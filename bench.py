@@ -10,7 +10,8 @@ Metric: Mpoint-steps/s = nx*ny*nz*steps / seconds / 1e6, inputs resident in
 HBM.  The roofline entry reports the dominant kernel's algorithmic bytes per
 launch over its average HIP-event duration against the 8 TB/s HBM peak.  The
 CPU baseline is the FP64 oracle (the CPU restatement of the reference
-algorithm) timed for one step of the same grid on this host's cores.
+algorithm) timed for a few steps (--cpu-steps, default six) of the same grid on
+this host's cores.
 """
 import argparse
 import json
