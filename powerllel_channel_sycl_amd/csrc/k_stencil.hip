@@ -117,7 +117,7 @@ constexpr int correct_ty() {
 }
 
 // Stage 1: d = s + dt*F(s).  Stage 2: d = (s + dt*F(s) + d)/2.
-// A block owns a 64x8 (i,j) tile and marches a chunk of k planes.  The three
+// A block owns a 64 x rk_ty (i,j) tile and marches a chunk of k planes.  The three
 // source fields of planes k-1, k, k+1 (tile plus a one-cell halo) sit in an
 // LDS ring; plane k+2 is fetched into registers while plane k is computed and
 // replaces plane k-1 afterwards, so every source value leaves HBM once per block.
@@ -983,7 +983,7 @@ inline T* F(pll_context* c, int f) {
   return static_cast<T*>(c->fld[f]);
 }
 
-// Tiled launch shape of the stencil kernels: ntx*nty tiles of 64x8 columns,
+// Tiled launch shape of the stencil kernels: ntx*nty tiles of 64 x ty columns,
 // each marching a chunk of k planes; about 2048 blocks in total.
 struct TileGrid {
   int ntx, nty, nkc, kchunk;
