@@ -64,7 +64,7 @@ STEP_BYTES_FP64 = 312
 # profiles/retau550_768x384x384_step_kernels.md holds the runs.  Reported as
 # the roofline entry's `traffic` for that workload only.
 MEASURED_HBM_B_PER_POINT = {
-    "fp64": {"rk1": 54.5, "rk2_rhs": 87.9, "fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 38.7,
+    "fp64": {"rk1": 54.5, "rk2_rhs": 84.5, "fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 38.7,
              "fft_y_bwd": 16.0, "fft_x_bwd": 16.0, "correct_vel": 59.9},
     "fp32_mixed": {"rk1": 27.1, "rk2": 38.9, "fft_x_fwd_rhs": 20.8, "fft_y_fwd": 8.0,
                    "solve_trid": 22.7, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
