@@ -105,17 +105,17 @@ constexpr int kRkTx = 64;  // tile width along x (one wavefront per row)
 constexpr int kRkTy = 8;   // tile rows along y
 
 // Tile rows of the stage and projection kernels: 4 for the FP64 stage 1 and
-// projection (4-7 % faster than 8 at 768x384x384 on MI355X), 16 for the fused
-// FP64 stage 2 + calcRHS (8-10 % faster than 8: fewer y-halo rows per output
-// row), 8 elsewhere (the FP32 stages lose 10-28 % with 4).
+// projection (4-7 % faster than 8 at 768x384x384 on MI355X), 8 elsewhere (the
+// FP32 stages lose 10-28 % with 4 and 28 % with 16).  The fused FP64 stage 2 +
+// calcRHS takes 16 rows where that still gives kRhsTallBlocks blocks (8-10 %
+// faster than 8 at 768x384x384: fewer y-halo rows per output row) and 8 on
+// smaller grids, where 16 would leave too few blocks (4 % slower at 96x64x64).
 template <typename T, int STAGE>
 constexpr int rk_ty() {
   return std::is_same<T, double>::value && STAGE == 1 ? 4 : kRkTy;
 }
-template <typename T>
-constexpr int rhs_ty() {
-  return std::is_same<T, double>::value ? 16 : kRkTy;
-}
+constexpr int kRhsTallTy = 16;
+constexpr long kRhsTallBlocks = 1024;
 template <typename T>
 constexpr int correct_ty() {
   return std::is_same<T, double>::value ? 4 : kRkTy;
@@ -378,15 +378,15 @@ __global__ __launch_bounds__((kRkTx * rk_ty<T, STAGE>())) void k_rk(Geom g, cons
 // added to the point values of u.  The new state goes to separate buffers
 // (nu, nv, nw): the backward neighbours read old values of cells that other
 // blocks update, so the update cannot be in place.
-template <typename T>
-__global__ __launch_bounds__((kRkTx * rhs_ty<T>())) void k_rk2_rhs(
+template <typename T, int TY>
+__global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     Geom g, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
     const T* __restrict__ du, const T* __restrict__ dv, const T* __restrict__ dw,
     T* __restrict__ nu_out, T* __restrict__ nv_out, T* __restrict__ nw_out, T* __restrict__ p, MeshT<T> m,
     T dxi, T dyi, T rei, T dt, T ucrf, int kchunk, int ntx, int nty, int defer, T bc_u,
     const double* __restrict__ force, int forced_mask, const RkCoef* __restrict__ coef,
     RkUniform cu, int crf, T dti, T dtidxi, T dtidyi) {
-  constexpr int TX = kRkTx, TY = rhs_ty<T>(), SX = TX + 3, SY = TY + 3, PL = SX * SY, NT = TX * TY;
+  constexpr int TX = kRkTx, SX = TX + 3, SY = TY + 3, PL = SX * SY, NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
   __shared__ T sh[3][3][PL];            // u1, v1, w1 ring; cell (cx, cy) = (i0-1+cx, j0-1+cy)
   __shared__ T un[TY][TX + 1];          // u_new at i = i0 .. i0+TX of the tile rows
@@ -1104,7 +1104,11 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
 // RK stage 2 + calcRHS into the spare velocity buffers, then swap them in.
 template <typename T>
 int rk2_rhs_t(pll_context* c, int defer) {
-  const TileGrid t = tile_grid(c, rhs_ty<T>());
+  constexpr int kTall = std::is_same<T, double>::value ? kRhsTallTy : kRkTy;
+  const bool tall = (long)tile_grid(c, kTall).blocks() >= kRhsTallBlocks;
+  const int ty = tall ? kTall : kRkTy;
+  const TileGrid t = tile_grid(c, ty);
+  auto kernel = tall ? k_rk2_rhs<T, kTall> : k_rk2_rhs<T, kRkTy>;
   const MeshT<T> m = mesh_of<T>(c);
   const T bc_u = (T)(0.0 - c->u_crf);  // src/mod_updateBound.cc:80
   const RkCoef* coef = static_cast<const RkCoef*>(c->d_rkcoef) + (c->g.nz + 2);
@@ -1114,7 +1118,7 @@ int rk2_rhs_t(pll_context* c, int defer) {
   const double dtidyi = dti * c->dy_inv;
   {
     TimerScope ts(c, K_RK2_RHS);
-    hipLaunchKernelGGL((k_rk2_rhs<T>), dim3(t.blocks()), dim3(kRkTx * rhs_ty<T>()), 0, c->stream, c->g,
+    hipLaunchKernelGGL(kernel, dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
                        F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
                        F<T>(c, PLL_V), F<T>(c, PLL_W), static_cast<T*>(c->fld_alt[0]),
                        static_cast<T*>(c->fld_alt[1]), static_cast<T*>(c->fld_alt[2]),
