@@ -29,6 +29,9 @@ CASES = {
     "c4": (1536, 768, 512, 2 * math.pi, math.pi, 2.0, 2.0, 20000.0, 2.5e-4),
     # no dimension a multiple of the closure stage's 32 x 8 x 4 blocks
     "ragged": (40, 30, 10, 2 * math.pi, math.pi, 2.0, 1.5, 2800.0, 2e-3),
+    # ragged 64 x 16 tiles (nx = 3*64 + 8, ny = 5*16 + 10) with enough blocks
+    # (1056) for the fused FP64 stage 2 to take its 16-row tiles
+    "ragtile": (200, 90, 176, 2 * math.pi, math.pi, 2.0, 1.5, 2800.0, 2e-3),
 }
 
 

@@ -217,7 +217,8 @@ def test_init_flow_matches_oracle(ctxs, name):
             assert rel_err(interior(ch.download(fld)), interior(arr)) < 1e-14, (prof, fld)
 
 
-@pytest.mark.parametrize("name,steps", [("c1", 10), ("r235", 10), ("r3", 5), ("gen", 10), ("c2", 10)])
+@pytest.mark.parametrize("name,steps", [("c1", 10), ("r235", 10), ("r3", 5), ("gen", 10), ("c2", 10),
+                                        ("ragtile", 5)])
 def test_full_step_parity(name, steps):
     """10 RK2 steps from the same seeded state: u, v, w and p - <p> to 1e-11."""
     _print_case("full step", name, steps)
