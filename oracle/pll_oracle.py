@@ -13,7 +13,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "build", "libpll_oracle.so")
+_DEFAULT_LIB = os.path.join(_HERE, "build", "libpll_oracle.so")
+# PLL_ORACLE_LIB names another build of pll_oracle.cc (the mutation tests load
+# deliberately broken copies through it to show that the pins catch them).
+_LIB_PATH = os.environ.get("PLL_ORACLE_LIB") or _DEFAULT_LIB
 
 
 class OrcConfig(C.Structure):
@@ -41,7 +44,7 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(_LIB_PATH):
+    if _LIB_PATH == _DEFAULT_LIB and not os.path.exists(_LIB_PATH):
         build()
     L = C.CDLL(_LIB_PATH)
     vp = C.c_void_p
