@@ -7,11 +7,11 @@ halo/BC, RK2, halo/BC, Poisson RHS, FFT/tridiagonal Poisson solve, pressure
 BC, projection, constant-flow forcing, halo/BC).
 
 Metric: Mpoint-steps/s = nx*ny*nz*steps / seconds / 1e6, inputs resident in
-HBM.  The roofline entry reports the dominant kernel's algorithmic bytes per
+HBM; three windows of --steps steps are timed and the median is reported.  The roofline entry reports the dominant kernel's algorithmic bytes per
 launch over its average HIP-event duration against the 8 TB/s HBM peak.  The
 CPU baseline is the FP64 oracle (the CPU restatement of the reference
 algorithm) timed for a few steps (--cpu-steps, default six) of the same grid on
-this host's cores.
+this host's cores (CPU model, thread count and visible cores are reported).
 """
 import argparse
 import json
@@ -55,6 +55,9 @@ BYTES_PER_POINT = {
                    "fft_y_fwd": 8, "solve_trid": 8, "fft_y_bwd": 8, "fft_x_bwd": 8,
                    "correct_vel": 28, "mean_partial": 4, "force_add": 8},
 }
+# Timed windows of --steps steps each; the reported time is their median.
+WINDOWS = 3
+
 # The algorithm's compulsory bytes per point per RK2 step (unfused, FP64): the
 # roofline of the whole step; FP32-mixed halves it.
 STEP_BYTES_FP64 = 312
@@ -222,17 +225,37 @@ def cpu_baseline(wl, steps):
     o.init_flow(u, v, w, prof, noise_intensity=0.1)
     o.update_bound_vel(u, v, w)
     o.update_bound_p(p)
+    o.step(1, u, v, w, u1, v1, w1, p)  # untimed: pages the fields in
     t0 = time.perf_counter()
     o.step(steps, u, v, w, u1, v1, w1, p)
     sec = time.perf_counter() - t0
+    threads = int(po.lib().orc_num_threads())
     return {
         "value": nx * ny * nz * steps / sec / 1e6,
         "unit": "Mpoint-steps/s",
-        "cores": int(po.lib().orc_num_threads()),
+        "cores": threads,
         "kind": "port",
-        "sample": f"{steps} FP64 RK2 step(s) of the full {nx}x{ny}x{nz} grid, "
-                  f"{sec:.2f} s, OpenMP oracle",
+        "sample": f"{steps} FP64 RK2 step(s) of the full {nx}x{ny}x{nz} grid after one untimed "
+                  f"step, {sec:.2f} s, OpenMP oracle (the reference's SYCL build cannot be "
+                  f"built without a SYCL compiler and oneMKL)",
+        "cpu_model": cpu_model(),
+        "omp_threads": threads,
+        "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+        "nproc": len(os.sched_getaffinity(0)),
+        "cpu_count": os.cpu_count(),
     }
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measure(wl, precision, steps, warmup):
@@ -242,12 +265,17 @@ def measure(wl, precision, steps, warmup):
           f"{ch.device_bytes() / 1e9:.2f} GB on device", flush=True)
     ch.step(warmup)
     ch.synchronize()
-    t0 = time.perf_counter()
-    ch.step(steps)
-    ch.synchronize()
-    sec = time.perf_counter() - t0
+    windows = []
+    for _ in range(WINDOWS):
+        t0 = time.perf_counter()
+        ch.step(steps)
+        ch.synchronize()
+        windows.append(time.perf_counter() - t0)
+    sec = sorted(windows)[len(windows) // 2]
     ms_step = 1e3 * sec / steps
-    print(f"# {precision}: timed {steps} steps, {ms_step:.3f} ms/step", flush=True)
+    print(f"# {precision}: {WINDOWS} windows of {steps} steps, "
+          f"{', '.join(f'{1e3 * w / steps:.3f}' for w in windows)} ms/step; median "
+          f"{ms_step:.3f}", flush=True)
     ch.timing(True)
     ch.timing_reset()
     ch.step(steps)
@@ -282,6 +310,7 @@ def measure(wl, precision, steps, warmup):
     return {
         "value": round(npts * steps / sec / 1e6, 2),
         "ms_per_step": round(ms_step, 4),
+        "window_ms_per_step": [round(1e3 * w / steps, 4) for w in windows],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "step_achieved": round(step_gbs, 1),
@@ -317,6 +346,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": main64["ms_per_step"],
+        "window_ms_per_step": main64["window_ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -332,6 +362,7 @@ def main():
         mixed = measure(args.workload, "fp32_mixed", args.steps, args.warmup)
         result["fp32_mixed"] = {
             "value": mixed["value"], "ms_per_step": mixed["ms_per_step"],
+            "window_ms_per_step": mixed["window_ms_per_step"],
             "dtype": "f32 fields/stencils/FFTs, f64 tridiagonal arithmetic and bulk reduction",
             "speedup_vs_fp64": round(mixed["value"] / main64["value"], 3),
             "roofline": mixed["roofline"], "kernels": mixed["kernels"], "check": mixed["check"],

@@ -109,7 +109,9 @@ int pll_create(const pll_config *cfg, const double *dzf_global, pll_context **ou
 /* free_poisson_solver + freeVariables + freeMesh (src/mod_poisson_solver.cc:512-537). */
 int pll_destroy(pll_context *ctx);
 int pll_get_config(const pll_context *ctx, pll_config *cfg);
-/* Bytes of device memory the context holds. */
+/* Bytes of device memory the context holds: every buffer it allocated (fields,
+ * spare fields, work array, pivots, twiddles, mesh, reduction scratch, and the
+ * statistics sums and closure buffers when enabled). */
 int pll_device_bytes(const pll_context *ctx, size_t *bytes);
 
 /* ---- field access (Array3D::data(), src/mod_type.hh:169-170) ----------------- */

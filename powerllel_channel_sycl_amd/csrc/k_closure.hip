@@ -1074,7 +1074,7 @@ int feat_setup(pll_context* c) {
     h[(nz + 2) + k] = (T)std::min(zc, lz - zc);
     h[2 * (nz + 2) + k] = (T)(zc < 0.5 * lz ? 1.0 : -1.0);
   }
-  PLL_HIP_TRY(hipMalloc(&c->d_feat, h.size() * sizeof(T)));
+  PLL_HIP_TRY(ctx_malloc(c, &c->d_feat, h.size() * sizeof(T)));
   PLL_HIP_TRY(hipMemcpy(c->d_feat, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   return PLL_OK;
 }
@@ -1198,7 +1198,7 @@ namespace pll {
 int launch_rk_closure(pll_context* c, int stage) {
   int rc = feat_setup_any(c);
   if (rc) return rc;
-  if (!c->d_nut) PLL_HIP_TRY(hipMalloc(&c->d_nut, (size_t)c->nspec * sizeof(float)));
+  if (!c->d_nut) PLL_HIP_TRY(ctx_malloc(c, &c->d_nut, (size_t)c->nspec * sizeof(float)));
   return c->prec == PLL_FP64 ? rk_closure_t<double>(c, stage) : rk_closure_t<float>(c, stage);
 }
 }  // namespace pll
@@ -1275,7 +1275,7 @@ int pll_closure_attach(pll_context* ctx, pll_closure* c) {
   if (c) {
     int rc = feat_setup_any(ctx);
     if (rc) return rc;
-    if (!ctx->d_nut) PLL_HIP_TRY(hipMalloc(&ctx->d_nut, (size_t)ctx->nspec * sizeof(float)));
+    if (!ctx->d_nut) PLL_HIP_TRY(ctx_malloc(ctx, &ctx->d_nut, (size_t)ctx->nspec * sizeof(float)));
     PLL_HIP_TRY(hipMemsetAsync(ctx->d_nut, 0, (size_t)ctx->nspec * sizeof(float), ctx->stream));
   }
   ctx->closure = c;

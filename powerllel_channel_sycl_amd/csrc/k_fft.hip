@@ -750,10 +750,10 @@ bool build_plan(int L, FftPlan& pl, std::vector<std::complex<long double>>& tw) 
 }
 
 template <typename T>
-int upload_table(const std::vector<std::complex<long double>>& src, void** dst) {
+int upload_table(pll_context* c, const std::vector<std::complex<long double>>& src, void** dst) {
   std::vector<cpx<T>> h(src.size());
   for (size_t t = 0; t < src.size(); ++t) h[t] = cpx<T>{(T)src[t].real(), (T)src[t].imag()};
-  PLL_HIP_TRY(hipMalloc(dst, h.size() * sizeof(cpx<T>)));
+  PLL_HIP_TRY(ctx_malloc(c, dst, h.size() * sizeof(cpx<T>)));
   PLL_HIP_TRY(hipMemcpy(*dst, h.data(), h.size() * sizeof(cpx<T>), hipMemcpyHostToDevice));
   return PLL_OK;
 }
@@ -772,9 +772,9 @@ int setup_t(pll_context* c) {
     rtw.emplace_back(cosl(ang), sinl(ang));
   }
   int rc;
-  if ((rc = upload_table<T>(twx, &c->d_tw_x))) return rc;
-  if ((rc = upload_table<T>(twy, &c->d_tw_y))) return rc;
-  if ((rc = upload_table<T>(rtw, &c->d_rtw_x))) return rc;
+  if ((rc = upload_table<T>(c, twx, &c->d_tw_x))) return rc;
+  if ((rc = upload_table<T>(c, twy, &c->d_tw_y))) return rc;
+  if ((rc = upload_table<T>(c, rtw, &c->d_rtw_x))) return rc;
   const size_t rowbytes = (size_t)nx * sizeof(T) * 2;  // two buffers
   long batch = (long)(kLdsBudget / rowbytes);
   if (batch < 1) return fail(PLL_ERR_ARG, "nx too large for one LDS row transform");
@@ -946,14 +946,14 @@ int y_t(pll_context* c, bool fwd) {
 
 // W_L^t = exp(-2 pi i t / L), t = 0..L-1, for the fixed-length kernels.
 template <typename T>
-int unit_table(int L, void** dst) {
+int unit_table(pll_context* c, int L, void** dst) {
   const long double two_pi = 6.283185307179586476925286766559005768L;
   std::vector<std::complex<long double>> t;
   for (int k = 0; k < L; ++k) {
     const long double ang = -two_pi * (long double)k / (long double)L;
     t.emplace_back(cosl(ang), sinl(ang));
   }
-  return upload_table<T>(t, dst);
+  return upload_table<T>(c, t, dst);
 }
 
 template <typename T>
@@ -963,8 +963,8 @@ int setup_fixed(pll_context* c) {
   c->x_fixed = !force_generic && x_has_fixed<T>(c->g.nx / 2);
   c->y_fixed = !force_generic && y_has_fixed<T>(c, c->g.ny);
   int rc;
-  if ((rc = unit_table<T>(c->g.nx / 2, &c->d_twl_x))) return rc;
-  if ((rc = unit_table<T>(c->g.ny, &c->d_twl_y))) return rc;
+  if ((rc = unit_table<T>(c, c->g.nx / 2, &c->d_twl_x))) return rc;
+  if ((rc = unit_table<T>(c, c->g.ny, &c->d_twl_y))) return rc;
   return PLL_OK;
 }
 

@@ -98,7 +98,7 @@ int pll_stat_enable(pll_context* c, const int which[11]) {
   const size_t bytes = (size_t)c->nspec * sizeof(double);
   for (int t = 0; t < 11; ++t) {
     if (which[t] && !c->d_stat[t]) {
-      hipError_t e = hipMalloc(&c->d_stat[t], bytes);
+      hipError_t e = ctx_malloc(c, &c->d_stat[t], bytes);
       if (e != hipSuccess) {
         c->d_stat[t] = nullptr;
         return fail(PLL_ERR_NOMEM, "statistics sums do not fit in device memory");
@@ -107,6 +107,7 @@ int pll_stat_enable(pll_context* c, const int which[11]) {
       PLL_HIP_TRY(hipStreamSynchronize(c->stream));
       (void)hipFree(c->d_stat[t]);
       c->d_stat[t] = nullptr;
+      c->dev_bytes -= bytes;
     }
     if (c->d_stat[t]) PLL_HIP_TRY(hipMemsetAsync(c->d_stat[t], 0, bytes, c->stream));
   }

@@ -1050,7 +1050,7 @@ int rk_coef_setup(pll_context* c) {
       q.cdw = (float)(vxy + (double)q.dp + (double)q.dm);
     }
   }
-  PLL_HIP_TRY(hipMalloc(&c->d_rkcoef, h.size() * sizeof(RkCoef)));
+  PLL_HIP_TRY(ctx_malloc(c, &c->d_rkcoef, h.size() * sizeof(RkCoef)));
   PLL_HIP_TRY(hipMemcpy(c->d_rkcoef, h.data(), h.size() * sizeof(RkCoef), hipMemcpyHostToDevice));
   return PLL_OK;
 }

@@ -180,9 +180,9 @@ int trid_setup(pll_context* c) {
   if (c->prec != PLL_FP64)
     for (double& x : piv) x = x != 0.0 ? 1.0 / x : 0.0;
 
-  PLL_HIP_TRY(hipMalloc(&c->d_a, nz * sizeof(double)));
-  PLL_HIP_TRY(hipMalloc(&c->d_c, nz * sizeof(double)));
-  PLL_HIP_TRY(hipMalloc(&c->d_piv, piv.size() * sizeof(double)));
+  PLL_HIP_TRY(ctx_malloc(c, &c->d_a, nz * sizeof(double)));
+  PLL_HIP_TRY(ctx_malloc(c, &c->d_c, nz * sizeof(double)));
+  PLL_HIP_TRY(ctx_malloc(c, &c->d_piv, piv.size() * sizeof(double)));
   PLL_HIP_TRY(hipMemcpy(c->d_a, a.data(), nz * sizeof(double), hipMemcpyHostToDevice));
   PLL_HIP_TRY(hipMemcpy(c->d_c, cc.data(), nz * sizeof(double), hipMemcpyHostToDevice));
   PLL_HIP_TRY(hipMemcpy(c->d_piv, piv.data(), piv.size() * sizeof(double), hipMemcpyHostToDevice));

@@ -194,8 +194,8 @@ int upload_mesh(pll_context* c) {
     m64[3 * n + k] = c->dzflzi[k];
   }
   for (long t = 0; t < 4 * n; ++t) m[t] = (T)m64[t];
-  PLL_HIP_TRY(hipMalloc(&c->d_mesh, m.size() * sizeof(T)));
-  PLL_HIP_TRY(hipMalloc(&c->d_mesh64, m64.size() * sizeof(double)));
+  PLL_HIP_TRY(ctx_malloc(c, &c->d_mesh, m.size() * sizeof(T)));
+  PLL_HIP_TRY(ctx_malloc(c, &c->d_mesh64, m64.size() * sizeof(double)));
   PLL_HIP_TRY(hipMemcpy(c->d_mesh, m.data(), m.size() * sizeof(T), hipMemcpyHostToDevice));
   PLL_HIP_TRY(hipMemcpy(c->d_mesh64, m64.data(), m64.size() * sizeof(double), hipMemcpyHostToDevice));
   return PLL_OK;
@@ -246,34 +246,33 @@ int create_impl(const pll_config* cfg, const double* dzf_global, pll_context** o
   if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreateWithFlags"));
   init_mesh(c, dzf_global);
   for (int f = 0; f < PLL_NFIELD; ++f) {
-    e = hipMalloc(&c->fld[f], c->nfield * c->esz);
+    e = ctx_malloc(c, &c->fld[f], c->nfield * c->esz);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(field)"));
     e = hipMemsetAsync(c->fld[f], 0, c->nfield * c->esz, c->stream);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMemsetAsync(field)"));
   }
   for (int d = 0; d < 3; ++d) {
-    e = hipMalloc(&c->fld_alt[d], c->nfield * c->esz);
+    e = ctx_malloc(c, &c->fld_alt[d], c->nfield * c->esz);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(spare field)"));
     e = hipMemsetAsync(c->fld_alt[d], 0, c->nfield * c->esz, c->stream);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMemsetAsync(spare field)"));
   }
-  e = hipMalloc(&c->spec, c->nspec * c->esz);
+  e = ctx_malloc(c, &c->spec, c->nspec * c->esz);
   if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(spec)"));
   rc = c->prec == PLL_FP64 ? upload_mesh<double>(c) : upload_mesh<float>(c);
   if (rc) return bail(rc);
   // reduction scratch: mean partials (chunks x nz), scalars, check partials
-  int chunks = 1;
-  while (chunks * 2 <= c->g.ny && (long)chunks * c->g.nz < 2048) chunks *= 2;
+  const int chunks = mean_chunks_for(c->g.ny, c->g.nz);
   c->mean_chunks = chunks;
   c->npartial = std::max<long>((long)chunks * c->g.nz, 1L << 15);
-  if ((e = hipMalloc(&c->d_partial, c->npartial * sizeof(double))) != hipSuccess)
+  if ((e = ctx_malloc(c, &c->d_partial, c->npartial * sizeof(double))) != hipSuccess)
     return bail(hip_fail(e, "hipMalloc(partial)"));
-  if ((e = hipMalloc(&c->d_scalar, 64 * sizeof(double))) != hipSuccess)
+  if ((e = ctx_malloc(c, &c->d_scalar, 64 * sizeof(double))) != hipSuccess)
     return bail(hip_fail(e, "hipMalloc(scalar)"));
   if ((e = hipMemsetAsync(c->d_scalar, 0, 64 * sizeof(double), c->stream)) != hipSuccess)
     return bail(hip_fail(e, "hipMemsetAsync(scalar)"));
   c->nred = 2048 * 16;
-  if ((e = hipMalloc(&c->d_red, c->nred)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(red)"));
+  if ((e = ctx_malloc(c, &c->d_red, c->nred)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(red)"));
   if ((rc = fft_setup(c))) return bail(rc);
   if ((rc = trid_setup(c))) return bail(rc);
   if ((rc = rk_setup(c))) return bail(rc);
@@ -466,9 +465,7 @@ int pll_get_config(const pll_context* ctx, pll_config* cfg) {
 
 int pll_device_bytes(const pll_context* c, size_t* bytes) {
   if (!c || !bytes) return fail(PLL_ERR_ARG, "null argument");
-  size_t b = PLL_NFIELD * (size_t)c->nfield * c->esz + (size_t)c->nspec * c->esz;
-  b += (size_t)c->cxn * c->cyn * c->g.nz * sizeof(double);
-  *bytes = b;
+  *bytes = c->dev_bytes;
   return PLL_OK;
 }
 

@@ -44,6 +44,15 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n) {
   return x * q + (x < r ? x : r) + l;
 }
 
+// Row chunks per plane of the bulk-mean reduction: the largest power of two
+// <= ny with chunks*nz reaching 2048 partial sums (fewer when ny is small).
+// Fewer than 4096 partials unless nz alone exceeds that (then nz).
+inline int mean_chunks_for(int ny, int nz) {
+  int chunks = 1;
+  while (chunks * 2 <= ny && (long)chunks * nz < 2048) chunks *= 2;
+  return chunks;
+}
+
 // Kernel identifiers for the per-kernel timers.
 enum KernelId {
   K_RK1 = 0,
@@ -137,6 +146,9 @@ struct pll_context {
   void* d_feat = nullptr;   // feature coefficients per plane (field precision), 3*(nz+2)
   float* d_nut = nullptr;   // eddy viscosity of the last closure stage, nx*ny*nz
 
+  // device bytes allocated for this context (ctx_malloc)
+  size_t dev_bytes = 0;
+
   // timers
   bool timing = false;
   std::vector<pll::PendingTimer> pending;
@@ -158,6 +170,14 @@ const char* last_error_cstr();
     hipError_t _e = (expr);                                 \
     if (_e != hipSuccess) return ::pll::hip_fail(_e, #expr); \
   } while (0)
+
+// hipMalloc of a buffer the context owns, counted in pll_device_bytes.
+template <typename P>
+hipError_t ctx_malloc(pll_context* c, P** p, size_t bytes) {
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  if (e == hipSuccess) c->dev_bytes += bytes;
+  return e;
+}
 
 // ---- timers -------------------------------------------------------------------
 void timer_begin(pll_context* c, int kid, PendingTimer* out);

@@ -3,6 +3,8 @@
 //   pll_channel [param.json]
 //   pll_channel --check [param.json]   parse, check and print the run set-up
 //                                      as JSON (no GPU needed)
+//   pll_channel --mesh [param.json]    parse, build the mesh and write its
+//                                      files as initialisation does (no GPU)
 //
 // Same initialisation order, step sequence, checks and screen output as the
 // reference.  With "fused_step" (default) the steps between two checks or
@@ -50,8 +52,11 @@ int next_event(int nt) {
   int e = nt_end;
   for (int every : {nt_check, nt_out_scrn, nt_out_inst, nt_out_save, nt_out_stat})
     e = std::min(e, next(every));
-  if (nt + 1 > nt_init_stat) e = nt + 1;
-  if (nt < nt_init_stat) e = std::min(e, nt_init_stat + 1);
+  // statistics start at nt_init_stat + 1 and sample every sample_interval
+  // steps after nt_init_stat (mod_statistics::calcStat); steps in between stay fused
+  if (nt <= nt_init_stat) e = std::min(e, nt_init_stat + 1);
+  else if (sample_interval > 0)
+    e = std::min(e, nt_init_stat + ((nt - nt_init_stat) / sample_interval + 1) * sample_interval);
   return e;
 }
 
@@ -205,14 +210,22 @@ int run(const char* fn) {
   return 0;
 }
 
+int mesh_only(const char* fn) {
+  readInputParam(fn);
+  mod_mesh::initMesh();
+  mod_mesh::outputMesh();
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-  const bool check = argc > 1 && std::string(argv[1]) == "--check";
-  const int a = check ? 2 : 1;
+  const std::string mode = argc > 1 ? argv[1] : "";
+  const bool check = mode == "--check", mesh = mode == "--mesh";
+  const int a = (check || mesh) ? 2 : 1;
   const char* fn = argc > a ? argv[a] : "param.json";
   try {
-    return check ? check_only(fn) : run(fn);
+    return check ? check_only(fn) : mesh ? mesh_only(fn) : run(fn);
   } catch (const std::exception& e) {
     std::fflush(stdout);
     std::fprintf(stderr, "%s\n", e.what());

@@ -145,6 +145,17 @@ void write_3d(hid_t f, const char* tag, const double* data) {
   if (e < 0) io_error(std::string("cannot write dataset ") + tag);
 }
 
+void write_1d(hid_t f, const char* tag, const double* data, long n) {
+  const H5Api& H = h5();
+  hsize_t dims = (hsize_t)n;
+  hid_t sp = H.screate_simple(1, &dims, nullptr);
+  hid_t ds = H.dcreate2(f, tag, *H.native_double, sp, kDefault, kDefault, kDefault);
+  herr_t e = ds < 0 ? -1 : H.dwrite(ds, *H.native_double, kAll, kAll, kDefault, data);
+  if (ds >= 0) H.dclose(ds);
+  H.sclose(sp);
+  if (e < 0) io_error(std::string("cannot write dataset ") + tag);
+}
+
 void read_3d(hid_t f, const char* tag, double* data) {
   const H5Api& H = h5();
   hid_t ds = H.dopen2(f, tag, kDefault);
@@ -214,6 +225,17 @@ void write_profiles(const std::string& name) {
 }
 
 }  // namespace
+
+void writeMesh(const char* filename) {
+  using mod_mesh::xc_global;
+  using mod_mesh::yc_global;
+  using mod_mesh::zc_global;
+  H5File f(filename, true);
+  // interior values, from index 1 (src/mod_hdf5.cc:124-137)
+  write_1d(f.id, "xc", &xc_global(1), nx);
+  write_1d(f.id, "yc", &yc_global(1), ny);
+  write_1d(f.id, "zc", &zc_global(1), nz);
+}
 
 void writeField(const char* filename, const char* tag, int nt, const Array3D& var, fp shift) {
   std::vector<double> buf = interior(var, shift);

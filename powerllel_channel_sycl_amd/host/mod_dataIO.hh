@@ -16,6 +16,10 @@ void outputData(int nt, fp u_crf, Array3DH3& u, const Array3DH3& v, const Array3
 void inputStatData(int nt_in_inst);
 void outputStatData(int nt);
 
+// The mesh file of outputMesh (src/mod_mesh.cc:127-136): datasets xc, yc, zc
+// of the interior cell-centre coordinates (nx, ny, nz values).
+void writeMesh(const char* filename);
+
 // One field file (exposed for tests): write the interior of var (host mirror
 // must be current) / read into the mirror's interior.
 void writeField(const char* filename, const char* tag, int nt, const Array3D& var, fp shift = 0.0);

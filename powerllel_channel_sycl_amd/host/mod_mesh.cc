@@ -4,6 +4,7 @@
 #include <fstream>
 #include <stdexcept>
 
+#include "mod_dataIO.hh"
 #include "mod_device.hh"
 #include "mod_parameters.hh"
 
@@ -74,6 +75,9 @@ void initMeshByFunc(int mtype, fp ratio, int nz_global, fp lz_, Array1DH1& dzf_g
 
 void outputMesh() {
   char name[64];
+  std::snprintf(name, sizeof(name), "mesh_%d-%d-%d.h5", nx, ny, nz);
+  mod_dataIO::writeMesh(name);
+  std::printf("PowerLLEL.NOTE.outputMesh: Finish writing file %s!\n", name);
   std::snprintf(name, sizeof(name), "mesh_%05d.out", nz);
   FILE* f = std::fopen(name, "w");
   if (!f) throw std::runtime_error(std::string("PowerLLEL.ERROR.outputMesh: cannot write ") + name);
@@ -81,6 +85,7 @@ void outputMesh() {
   for (int k = 1; k <= nz; ++k)
     std::fprintf(f, "%05d %23.15E %23.15E\n", k, dzf_global(k), zc_global(k));
   std::fclose(f);
+  std::printf("PowerLLEL.NOTE.outputMesh: Finish writing file %s!\n", name);
 }
 
 void freeMesh() {

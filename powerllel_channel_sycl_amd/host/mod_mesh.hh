@@ -21,7 +21,8 @@ void initMesh();
 // Reads nz_global spacings from mesh.in (src/mod_mesh.cc:78-93).
 void inputMesh(int nz_global, Array1DH1& dzf_global);
 void initMeshByFunc(int mesh_type, fp stretch_ratio, int nz_global, fp lz, Array1DH1& dzf_global);
-// Writes mesh_<nz>.out: k, dzf, zc (the text half of src/mod_mesh.cc:127-145).
+// Writes mesh_<nx>-<ny>-<nz>.h5 (datasets xc, yc, zc) and mesh_<nz>.out (k,
+// dzf, zc), src/mod_mesh.cc:127-145.
 void outputMesh();
 void freeMesh();
 }  // namespace mod_mesh

@@ -71,3 +71,29 @@ def test_unreadable_file(tmp_path):
     assert r.returncode == 2 and "readInputParam" in r.stderr
     r = du.run(tmp_path, "--check", "missing.json")
     assert r.returncode == 2 and "Fail to open file missing.json" in r.stderr
+
+
+def test_mesh_files(tmp_path, oracle_lib):
+    """outputMesh (src/mod_mesh.cc:127-145): mesh_<nx>-<ny>-<nz>.h5 holds the
+    interior cell centres xc, yc, zc; mesh_<nz>.out lists k, dzf, zc."""
+    import h5lite
+    if not h5lite.available():
+        pytest.skip("no HDF5 library")
+    r = du.run(tmp_path, "--mesh", du.write(tmp_path, du.param()))
+    assert r.returncode == 0, r.stderr
+    m = du.BASE["MESH"]
+    nx, ny, nz = m["nx"], m["ny"], m["nz"]
+    name = f"mesh_{nx}-{ny}-{nz}.h5"
+    assert f"Finish writing file {name}" in r.stdout
+    with h5lite.read(str(tmp_path / name)) as f:
+        xc, yc, zc = (f.dataset(t) for t in ("xc", "yc", "zc"))
+    dx, dy = m["lx"] / nx, m["ly"] / ny
+    np.testing.assert_allclose(xc, (np.arange(1, nx + 1) - 0.5) * dx, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(yc, (np.arange(1, ny + 1) - 0.5) * dy, rtol=0, atol=1e-14)
+    dzf = oracle_lib.mesh_by_func(1, 0, m["stretch_ratio"], nz, m["lz"])
+    o = oracle_lib.Oracle(nx, ny, nz, m["lx"], m["ly"], m["lz"], 2800.0, 2e-3, dzf)
+    np.testing.assert_array_equal(zc, o.mesh()["zc_global"][1:nz + 1])
+    rows = (tmp_path / f"mesh_{nz:05d}.out").read_text().splitlines()
+    assert len(rows) == nz + 1
+    k, dz, z = rows[-1].split()
+    assert int(k) == nz and abs(float(dz) - dzf[nz]) < 1e-14 and abs(float(z) - zc[-1]) < 1e-14
