@@ -38,17 +38,17 @@ WORKLOADS = {
 }
 DEFAULT_WORKLOAD = "retau550_768x384x384"
 
-# Algorithmic (compulsory) HBM bytes per grid point per launch.  FP64: RK1
-# reads u,v,w and writes u1,v1,w1; RK2 fused with calcRHS reads u1,v1,w1,u,v,w
-# and writes u,v,w,p; each transform and the tridiagonal solve read and write
-# the work array once; correctVel reads p,u,v,w and writes u,v,w (the bulk mean
-# of u rides along); the separate bulk mean reads u; the force update reads and
-# writes u.  FP32-mixed (4-byte fields): stage 1 also reads p (the previous
-# step's projection is applied on load), stage 2 reads u1,v1,w1,u,v,w,p and
-# writes u,v,w, and calcRHS runs inside the forward x transform (reads u,v,w,
-# writes the work array).
+# Algorithmic (compulsory) HBM bytes per grid point per launch.  Inside a run
+# the previous step's projection is applied on load, so stage 1 reads u,v,w,p
+# and writes u1,v1,w1.  FP64: stage 2 fused with calcRHS reads u1,v1,w1,u,v,w,p
+# and writes u,v,w and the right-hand side; each transform and the tridiagonal
+# solve read and write the work array once; correctVel (last step of a run)
+# reads p,u,v,w and writes u,v,w (the bulk mean of u rides along); the separate
+# bulk mean reads u; the force update reads and writes u.  FP32-mixed (4-byte
+# fields): stage 2 reads u1,v1,w1,u,v,w,p and writes u,v,w, and calcRHS runs
+# inside the forward x transform (reads u,v,w, writes the work array).
 BYTES_PER_POINT = {
-    "fp64": {"rk1": 48, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 80, "fft_x_fwd": 16,
+    "fp64": {"rk1": 56, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 88, "fft_x_fwd": 16,
              "fft_y_fwd": 16, "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16,
              "correct_vel": 56, "mean_partial": 8, "force_add": 16},
     "fp32_mixed": {"rk1": 28, "rk2": 40, "calc_rhs": 16, "fft_x_fwd": 8, "fft_x_fwd_rhs": 16,
@@ -67,8 +67,8 @@ STEP_BYTES_FP64 = 312
 # profiles/retau550_768x384x384_step_kernels.md holds the runs.  Reported as
 # the roofline entry's `traffic` for that workload only.
 MEASURED_HBM_B_PER_POINT = {
-    "fp64": {"rk1": 54.5, "rk2_rhs": 84.5, "fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 38.7,
-             "fft_y_bwd": 16.0, "fft_x_bwd": 16.0, "correct_vel": 59.9},
+    "fp64": {"fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 38.7, "fft_y_bwd": 16.0,
+             "fft_x_bwd": 16.0, "correct_vel": 59.9},
     "fp32_mixed": {"rk1": 27.1, "rk2": 38.9, "fft_x_fwd_rhs": 20.8, "fft_y_fwd": 8.0,
                    "solve_trid": 22.7, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
 }

@@ -862,14 +862,15 @@ int x_fixed(pll_context* c, bool fwd, bool rhs = false) {
   } else {
     const double norm = 1.0 / (c->g.nx * c->g.ny);  // src/mod_fft.cc:57
     TimerScope ts(c, K_FFT_X_BWD);
-    if (rhs)  // inside the step: updateBoundP follows, x halos may be written
+    if (rhs) {  // inside the step: updateBoundP follows, x halos may be written
       hipLaunchKernelGGL((k_xb<T, L, true>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
                          static_cast<const T*>(c->spec), static_cast<T*>(c->fld[PLL_P]), tw, rtw,
                          nrows, (T)norm);
-    else
+    } else {
       hipLaunchKernelGGL((k_xb<T, L, false>), dim3(blocks), dim3(NT), 0, c->stream, c->g,
                          static_cast<const T*>(c->spec), static_cast<T*>(c->fld[PLL_P]), tw, rtw,
                          nrows, (T)norm);
+    }
   }
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
@@ -1011,6 +1012,15 @@ int launch_fft_x_fwd_rhs(pll_context* c) {
   return c->prec == PLL_FP64 ? x_rhs_t<double>(c) : x_rhs_t<float>(c);
 }
 int launch_fft_x_bwd(pll_context* c) { return c->prec == PLL_FP64 ? x_t<double>(c, false) : x_t<float>(c, false); }
+
+int launch_poisson_step(pll_context* c, bool rhs) {
+  int rc;
+  if ((rc = rhs ? launch_fft_x_fwd_rhs(c) : launch_fft_x_fwd(c))) return rc;
+  if ((rc = launch_fft_y_fwd(c))) return rc;
+  if ((rc = launch_trid(c))) return rc;
+  if ((rc = launch_fft_y_bwd(c))) return rc;
+  return launch_fft_x_bwd_step(c);
+}
 int launch_fft_y_fwd(pll_context* c) { return c->prec == PLL_FP64 ? y_t<double>(c, true) : y_t<float>(c, true); }
 int launch_fft_y_bwd(pll_context* c) { return c->prec == PLL_FP64 ? y_t<double>(c, false) : y_t<float>(c, false); }
 

@@ -104,17 +104,16 @@ __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W
 constexpr int kRkTx = 64;  // tile width along x (one wavefront per row)
 constexpr int kRkTy = 8;   // tile rows along y
 
-// Tile rows of the stage and projection kernels: 4 for the FP64 stage 1 and
-// projection (4-7 % faster than 8 at 768x384x384 on MI355X), 8 elsewhere (the
-// FP32 stages lose 10-28 % with 4 and 28 % with 16).  The fused FP64 stage 2 +
-// calcRHS takes 16 rows where that still gives kRhsTallBlocks blocks (8-10 %
-// faster than 8 at 768x384x384: fewer y-halo rows per output row) and 8 on
-// smaller grids, where 16 would leave too few blocks (4 % slower at 96x64x64).
-template <typename T, int STAGE>
-constexpr int rk_ty() {
-  return std::is_same<T, double>::value && STAGE == 1 ? 4 : kRkTy;
-}
-constexpr int kRhsTallTy = 16;
+// Tile rows of the stage and projection kernels: 4 for the FP64 projection
+// (4-7 % faster than 8 at 768x384x384 on MI355X), 8 elsewhere (the FP32 stages
+// lose 10-28 % with 4 and 28 % with 16; the FP64 stage 1, which applies the
+// deferred projection on load, is 4 % faster with 8 than with 4).  The fused FP64 stage 2 +
+// calcRHS takes 12 rows where that still gives kRhsTallBlocks blocks (fewer
+// y-halo rows per output row than 8) and 8 on smaller grids, where taller tiles
+// would leave too few blocks (4 % slower at 96x64x64).  12 rows (768 threads,
+// up to 168 VGPRs, one block per CU) rather than 16: at 1024 threads its
+// deferred-projection variant is held to 128 VGPRs and spills.
+constexpr int kRhsTallTy = 12;
 constexpr long kRhsTallBlocks = 1024;
 template <typename T>
 constexpr int correct_ty() {
@@ -122,7 +121,7 @@ constexpr int correct_ty() {
 }
 
 // Stage 1: d = s + dt*F(s).  Stage 2: d = (s + dt*F(s) + d)/2.
-// A block owns a 64 x rk_ty (i,j) tile and marches a chunk of k planes.  The three
+// A block owns a 64 x 8 (i,j) tile and marches a chunk of k planes.  The three
 // source fields of planes k-1, k, k+1 (tile plus a one-cell halo) sit in an
 // LDS ring; plane k+2 is fetched into registers while plane k is computed and
 // replaces plane k-1 afterwards, so every source value leaves HBM once per block.
@@ -144,8 +143,11 @@ constexpr int correct_ty() {
 // stage 2 also sums its new u times dzflzi(k) in FP64 into partial[block]: the
 // bulk mean of the next projected velocity, since the projection changes every
 // x-row's sum only by p(nx+1) - p(1) = 0.
-template <typename T, int STAGE>
-__global__ __launch_bounds__((kRkTx * rk_ty<T, STAGE>())) void k_rk(Geom g, const T* __restrict__ su,
+// PROJ compiles the deferred projection in: always in FP32; in FP64 only the
+// stage-1 variant that needs it, so the other FP64 launches keep their
+// register count and occupancy.
+template <typename T, int STAGE, bool PROJ = std::is_same<T, float>::value>
+__global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restrict__ su,
                                                      const T* __restrict__ sv,
                                                      const T* __restrict__ sw, T* __restrict__ du,
                                                      T* __restrict__ dv, T* __restrict__ dw,
@@ -160,13 +162,13 @@ __global__ __launch_bounds__((kRkTx * rk_ty<T, STAGE>())) void k_rk(Geom g, cons
                                                      T dtdxi, T dtdyi, int acc,
                                                      const double* __restrict__ dzflzi,
                                                      double* __restrict__ partial, int bcload) {
-  constexpr int TX = kRkTx, TY = rk_ty<T, STAGE>(), SX = TX + 2, SY = TY + 2, PL = SX * SY,
+  constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY,
                 NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
-  // the deferred projection and the predictor's mean serve the FP32-mixed step
-  // only; compiling them out of FP64 keeps its register count and occupancy
+  // the predictor's mean of stage 2 serves the FP32-mixed step only (the FP64
+  // step sums it in the fused stage 2 + calcRHS kernel)
   constexpr bool kF32 = std::is_same<T, float>::value;
-  corr = kF32 ? corr : 0;
+  corr = PROJ ? corr : 0;
   acc = kF32 ? acc : 0;
   // halo cells from the boundary rules on load: with the folded forcing, or
   // alone (bcload) for a source whose boundary update was not run
@@ -378,17 +380,30 @@ __global__ __launch_bounds__((kRkTx * rk_ty<T, STAGE>())) void k_rk(Geom g, cons
 // added to the point values of u.  The new state goes to separate buffers
 // (nu, nv, nw): the backward neighbours read old values of cells that other
 // blocks update, so the update cannot be in place.
-template <typename T, int TY>
+//
+// With PROJ the previous step's projection is deferred into this kernel as
+// well: the stored u, v, w are the predictor u* and every point value read is
+// u* - dt*grad(p) (src/mod_calcVel.cc:272-274, k_correct's association) before
+// the force applies; p (pin, valid halos) is the previous pressure, held in a
+// fourth field of the LDS ring, and the right-hand side goes to a separate
+// buffer (pout).  With `acc` the block sums
+// its new u times dzflzi(k) in FP64 into partial[block]: the bulk mean of the
+// next projected velocity (the projection changes every x-row's sum only by
+// p(nx+1) - p(1) = 0).
+template <typename T, int TY, bool PROJ>
 __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     Geom g, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
     const T* __restrict__ du, const T* __restrict__ dv, const T* __restrict__ dw,
-    T* __restrict__ nu_out, T* __restrict__ nv_out, T* __restrict__ nw_out, T* __restrict__ p, MeshT<T> m,
-    T dxi, T dyi, T rei, T dt, T ucrf, int kchunk, int ntx, int nty, int defer, T bc_u,
-    const double* __restrict__ force, int forced_mask, const RkCoef* __restrict__ coef,
-    RkUniform cu, int crf, T dti, T dtidxi, T dtidyi) {
+    T* __restrict__ nu_out, T* __restrict__ nv_out, T* __restrict__ nw_out,
+    const T* __restrict__ pin, T* __restrict__ pout, MeshT<T> m, T dxi, T dyi, T rei, T dt,
+    T ucrf, int kchunk, int ntx, int nty, int defer, T bc_u, const double* __restrict__ force,
+    int forced_mask, const RkCoef* __restrict__ coef, RkUniform cu, int crf, T dti, T dtidxi,
+    T dtidyi, T pdx, T pdy, int acc, const double* __restrict__ dzflzi,
+    double* __restrict__ partial) {
   constexpr int TX = kRkTx, SX = TX + 3, SY = TY + 3, PL = SX * SY, NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
-  __shared__ T sh[3][3][PL];            // u1, v1, w1 ring; cell (cx, cy) = (i0-1+cx, j0-1+cy)
+  constexpr int NF = PROJ ? 4 : 3;
+  __shared__ T sh[NF][3][PL];  // u1, v1, w1 (and p) ring; cell (cx, cy) = (i0-1+cx, j0-1+cy)
   __shared__ T un[TY][TX + 1];          // u_new at i = i0 .. i0+TX of the tile rows
   __shared__ T vn[TY + 1][TX];          // v_new at j = j0 .. j0+TY of the tile columns
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
@@ -416,8 +431,11 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     const bool inside = e < PL && gx <= g.nx + 1 && gy <= g.ny + 1;
     gaddr[n] = inside ? wrapx(gx) + g.sx * wrapy(gy) : -1;
   }
-  T pre[3][NPF];
-  auto fetch = [&](int kk) {  // u1, v1, w1 of plane kk with the boundary rules applied
+  T pre[NF][NPF];
+  // u1, v1, w1 of plane kk with the boundary rules applied; p of plane kk as
+  // stored (updateBoundP ran; x/y halo cells read their periodic image, which
+  // holds the same value)
+  auto fetch = [&](int kk) {
     const bool ghost = kk == 0 || kk == g.nz + 1;
     const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
     const bool wzero = kk == 0 || kk >= g.nz;
@@ -434,24 +452,32 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
         pre[0][n] = qu;
         pre[1][n] = qv;
         pre[2][n] = qw;
+        if constexpr (PROJ) pre[NF - 1][n] = pin[gaddr[n] + g.sxy * kk];
       }
   };
   auto commit = [&](int slot) {
 #pragma unroll
     for (int n = 0; n < NPF; ++n)
       if (gaddr[n] >= 0) {
-        sh[0][slot][cell[n]] = pre[0][n];
-        sh[1][slot][cell[n]] = pre[1][n];
-        sh[2][slot][cell[n]] = pre[2][n];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sh[f][slot][cell[n]] = pre[f][n];
       }
   };
-  // old (u, v, w) at a cell, with the deferred force; x/y halo cells wrap
-  auto old_u = [&](int x, int y, int kk) {
+  // the projection's pressure differences at tile cell c of ring slots s0
+  // (plane k) and sp (plane k+1)
+  auto dpx = [&](int s0, int c) { return sh[NF - 1][s0][c + 1] - sh[NF - 1][s0][c]; };
+  auto dpy = [&](int s0, int c) { return sh[NF - 1][s0][c + SX] - sh[NF - 1][s0][c]; };
+  auto dpz = [&](int s0, int sp, int c) { return sh[NF - 1][sp][c] - sh[NF - 1][s0][c]; };
+  // old u at tile cell (1, cy), old v at (cx, 1) of plane kk (slot s0), with
+  // the deferred projection and force; x/y halo cells wrap
+  auto old_u = [&](int s0, int cy, int x, int y, int kk) {
     T q = du[g.idx(wrapx(x), wrapy(y), kk)];
+    if constexpr (PROJ) q = q - dpx(s0, cy * SX + 1) * pdx;
     return fu ? q + f_u : q;
   };
-  auto old_v = [&](int x, int y, int kk) {
+  auto old_v = [&](int s0, int cx, int x, int y, int kk) {
     T q = dv[g.idx(wrapx(x), wrapy(y), kk)];
+    if constexpr (PROJ) q = q - dpy(s0, SX + cx) * pdy;
     return fv ? q + f_v : q;
   };
   // the three stage-2 updates at tile cell (cx, cy) of plane kk; `which` selects components
@@ -479,8 +505,12 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
       nw = (W(0, 0, 0) + ow) * 0.5f + iw;
     }
   };
-  if (k0 > k1) return;
+  if (k0 > k1) {
+    if (acc && threadIdx.x == 0) partial[b] = 0.0;
+    return;
+  }
   const int cxi = tx + 2, cyi = ty + 2;  // tile cell of the thread's interior point
+  double msum = 0.0;  // FP64 partial of the bulk mean (acc)
   int sm = 0, s0 = 1, sp = 2;
   T wprev = T(0);  // w_new(i, j, k-1); 0 below the wall (w(0) = 0)
   if (k0 > 1) {
@@ -495,6 +525,8 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     if (active) {
       T nu, nv, nw;
       T ow = dw[g.idx(i, j, k0 - 1)];
+      // same association as k_correct: ((p(k+1) - p(k)) * dt) * dzc_inv(k)
+      if constexpr (PROJ) ow = ow - dpz(1, 2, cyi * SX + cxi) * dt * m.dzc_inv[k0 - 1];
       if (fw) ow = (k0 - 1 == g.nz) ? T(0) : ow + f_w;
       update(0, 1, 2, cxi, cyi, k0 - 1, nu, nv, nw, T(0), T(0), ow);
       wprev = (k0 - 1 == g.nz) ? T(0) : nw;
@@ -522,6 +554,13 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     if (active) {
       const long a = g.idx(i, j, k);
       T ou = du[a], ov = dv[a], ow = dw[a];
+      if constexpr (PROJ) {
+        // same association as k_correct: ((p(k+1) - p(k)) * dt) * dzc_inv(k)
+        const int c = cyi * SX + cxi;
+        ou = ou - dpx(s0, c) * pdx;
+        ov = ov - dpy(s0, c) * pdy;
+        ow = ow - dpz(s0, sp, c) * dt * m.dzc_inv[k];
+      }
       if (fu) ou = ou + f_u;
       if (fv) ov = ov + f_v;
       if (fw) ow = (k == g.nz) ? T(0) : ow + f_w;
@@ -530,6 +569,7 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
       if (k == g.nz) nw = T(0);  // src/mod_updateBound.cc:137 (w(nz) = 0)
       un[ty][tx + 1] = nu;
       vn[ty + 1][tx] = nv;
+      if (acc) msum += (double)nu * dzflzi[k];
       nu_out[a] = nu;
       nv_out[a] = nv;
       nw_out[a] = nw;
@@ -537,18 +577,19 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     // backward neighbours: v_new on row j0 (wave 0), u_new on column i0 (wave 1)
     if (ty == 0 && i <= g.nx) {
       T nu, nv, nw2;
-      update(sm, s0, sp, cxi, 1, k, nu, nv, nw2, T(0), old_v(i, j0, k), T(0));
+      update(sm, s0, sp, cxi, 1, k, nu, nv, nw2, T(0), old_v(s0, cxi, i, j0, k), T(0));
       vn[0][tx] = nv;
     } else if (ty == 1 && tx < TY && j0 + tx + 1 <= g.ny) {
       T nu, nv, nw2;
-      update(sm, s0, sp, 1, tx + 2, k, nu, nv, nw2, old_u(i0, j0 + tx + 1, k), T(0), T(0));
+      update(sm, s0, sp, 1, tx + 2, k, nu, nv, nw2, old_u(s0, tx + 2, i0, j0 + tx + 1, k), T(0),
+             T(0));
       un[tx][0] = nu;
     }
     __syncthreads();
     if (active) {
       // calcRHS, src/mod_calcRHS.cc:27-29
-      p[g.idx(i, j, k)] = (un[ty][tx + 1] - un[ty][tx]) * dtidxi +
-                          (vn[ty + 1][tx] - vn[ty][tx]) * dtidyi + (nw - wprev) * dti * m.dzf_inv[k];
+      pout[g.idx(i, j, k)] = (un[ty][tx + 1] - un[ty][tx]) * dtidxi +
+                             (vn[ty + 1][tx] - vn[ty][tx]) * dtidyi + (nw - wprev) * dti * m.dzf_inv[k];
       wprev = nw;
     }
     if (more) commit(sm);
@@ -557,6 +598,23 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     sm = s0;
     s0 = sp;
     sp = t;
+  }
+  if (acc) {
+    // the ring is free after the last barrier of the loop; the block size
+    // (64 x TY) is folded onto its largest power of two first
+    double* red = reinterpret_cast<double*>(&sh[0][0][0]);
+    static_assert(sizeof(sh) >= NT * sizeof(double), "reduction scratch");
+    constexpr int P2 = NT >= 1024 ? 1024 : (NT >= 512 ? 512 : 256);
+    static_assert(NT >= P2 && NT < 2 * P2, "block size");
+    red[threadIdx.x] = msum;
+    __syncthreads();
+    if ((int)threadIdx.x < NT - P2) red[threadIdx.x] += red[threadIdx.x + P2];
+    __syncthreads();
+    for (int h = P2 / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[b] = red[0];
   }
 }
 
@@ -1057,7 +1115,7 @@ int rk_coef_setup(pll_context* c) {
 
 template <typename T>
 int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
-  const int ty = stage == 1 ? rk_ty<T, 1>() : rk_ty<T, 2>();
+  const int ty = kRkTy;
   const TileGrid t = tile_grid(c, ty);
   if (acc && (long)t.blocks() > c->npartial)
     return fail(PLL_ERR_STATE, "reduction scratch smaller than the stencil grid");
@@ -1073,8 +1131,10 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
   const RkUniform cu = rk_uniform(c, stage);
   const int crf = c->u_crf != 0.0;
   if (stage == 1) {
+    // FP64 compiles the deferred projection into its own variant of stage 1
+    auto kernel = corr ? k_rk<T, 1, true> : k_rk<T, 1>;
     TimerScope ts(c, K_RK1);
-    hipLaunchKernelGGL((k_rk<T, 1>), dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
+    hipLaunchKernelGGL(kernel, dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
                        F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
                        F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
                        (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
@@ -1101,14 +1161,23 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
   return PLL_OK;
 }
 
-// RK stage 2 + calcRHS into the spare velocity buffers, then swap them in.
+// RK stage 2 + calcRHS into the spare velocity and pressure buffers, then
+// swap them in.  corr: the previous step's projection is applied on load;
+// acc: the bulk mean of the new u and the force of u go to d_scalar[0], [3].
 template <typename T>
-int rk2_rhs_t(pll_context* c, int defer) {
+int rk2_rhs_t(pll_context* c, int defer, int corr, int acc) {
   constexpr int kTall = std::is_same<T, double>::value ? kRhsTallTy : kRkTy;
   const bool tall = (long)tile_grid(c, kTall).blocks() >= kRhsTallBlocks;
   const int ty = tall ? kTall : kRkTy;
   const TileGrid t = tile_grid(c, ty);
-  auto kernel = tall ? k_rk2_rhs<T, kTall> : k_rk2_rhs<T, kRkTy>;
+  if (acc && (long)t.blocks() > c->npartial)
+    return fail(PLL_ERR_STATE, "reduction scratch smaller than the stencil grid");
+  if (!c->fld_alt[3]) return fail(PLL_ERR_STATE, "no spare pressure buffer");
+  // the projection reads p at the periodic image of halo cells, which only the
+  // folded loads of stage 2 provide
+  if (corr && !defer) return fail(PLL_ERR_STATE, "deferred projection needs folded loads");
+  auto kernel = corr ? (tall ? k_rk2_rhs<T, kTall, true> : k_rk2_rhs<T, kRkTy, true>)
+                     : (tall ? k_rk2_rhs<T, kTall, false> : k_rk2_rhs<T, kRkTy, false>);
   const MeshT<T> m = mesh_of<T>(c);
   const T bc_u = (T)(0.0 - c->u_crf);  // src/mod_updateBound.cc:80
   const RkCoef* coef = static_cast<const RkCoef*>(c->d_rkcoef) + (c->g.nz + 2);
@@ -1116,19 +1185,31 @@ int rk2_rhs_t(pll_context* c, int defer) {
   const double dti = 1.0 / c->dt;  // src/mod_calcRHS.cc:11-13
   const double dtidxi = dti * c->dx_inv;
   const double dtidyi = dti * c->dy_inv;
+  const double pdx = c->dt * c->dx_inv, pdy = c->dt * c->dy_inv;  // src/mod_calcVel.cc:256-258
+  const double* dzflzi = c->d_mesh64 + 3 * (c->g.nz + 2);
   {
     TimerScope ts(c, K_RK2_RHS);
     hipLaunchKernelGGL(kernel, dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
                        F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
                        F<T>(c, PLL_V), F<T>(c, PLL_W), static_cast<T*>(c->fld_alt[0]),
                        static_cast<T*>(c->fld_alt[1]), static_cast<T*>(c->fld_alt[2]),
-                       F<T>(c, PLL_P), m, (T)c->dx_inv, (T)c->dy_inv, (T)c->re_inv, (T)c->dt,
-                       (T)c->u_crf, t.kchunk, t.ntx, t.nty, defer, bc_u, c->d_scalar + 3,
-                       forced_mask(c), coef, cu, (int)(c->u_crf != 0.0), (T)dti, (T)dtidxi,
-                       (T)dtidyi);
+                       F<T>(c, PLL_P), static_cast<T*>(c->fld_alt[3]), m, (T)c->dx_inv,
+                       (T)c->dy_inv, (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty,
+                       defer, bc_u, c->d_scalar + 3, forced_mask(c), coef, cu,
+                       (int)(c->u_crf != 0.0), (T)dti, (T)dtidxi, (T)dtidyi, (T)pdx, (T)pdy, acc,
+                       dzflzi, c->d_partial);
   }
   PLL_HIP_TRY(hipGetLastError());
   for (int d = 0; d < 3; ++d) std::swap(c->fld[PLL_U + d], c->fld_alt[d]);
+  std::swap(c->fld[PLL_P], c->fld_alt[3]);
+  if (acc) {
+    // forceVel's bulk mean and force of u (src/mod_calcVel.cc:287,297)
+    TimerScope ts(c, K_MEAN_FINAL);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kRedThreads), 0, c->stream, c->d_partial,
+                       (long)t.blocks(), (double)c->g.nx, (double)c->g.ny, c->d_scalar + 0,
+                       c->vel_force[0], c->d_scalar + 3);
+    PLL_HIP_TRY(hipGetLastError());
+  }
   return PLL_OK;
 }
 
@@ -1351,8 +1432,9 @@ int launch_rk(pll_context* c, int stage, int fold, int corr, int acc, int bcload
                       rk_t<float>(c, stage, fold, corr, acc, bcload));
 }
 int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
-int launch_rk2_rhs(pll_context* c, int defer) {
-  return PLL_DISPATCH(c, rk2_rhs_t<double>(c, defer), rk2_rhs_t<float>(c, defer));
+int launch_rk2_rhs(pll_context* c, int defer, int corr, int acc) {
+  if (c->prec != PLL_FP64) return fail(PLL_ERR_STATE, "the fused stage 2 + calcRHS is the FP64 step's");
+  return rk2_rhs_t<double>(c, defer, corr, acc);
 }
 int launch_correct_vel(pll_context* c, bool with_mean) {
   return PLL_DISPATCH(c, correct_t<double>(c, with_mean), correct_t<float>(c, with_mean));

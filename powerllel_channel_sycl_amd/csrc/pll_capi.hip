@@ -251,7 +251,9 @@ int create_impl(const pll_config* cfg, const double* dzf_global, pll_context** o
     e = hipMemsetAsync(c->fld[f], 0, c->nfield * c->esz, c->stream);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMemsetAsync(field)"));
   }
-  for (int d = 0; d < 3; ++d) {
+  // spare u, v, w (and p for the FP64 step's fused stage 2 + calcRHS)
+  const int nspare = c->prec == PLL_FP64 ? 4 : 3;
+  for (int d = 0; d < nspare; ++d) {
     e = ctx_malloc(c, &c->fld_alt[d], c->nfield * c->esz);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(spare field)"));
     e = hipMemsetAsync(c->fld_alt[d], 0, c->nfield * c->esz, c->stream);
@@ -292,11 +294,13 @@ int poisson_impl(pll_context* c) {
 }
 
 // One step of src/main.cc:116-154.  When only u is forced (or nothing is),
-// the bulk mean is reduced inside correctVel, and within a multi-step run the
-// force update and the final boundary update of a step (:149-153) are folded
-// into the next step's RK loads (`fold`); the last step of a run applies them
-// explicitly, so every run ends in exactly the reference's state.
+// within a multi-step run the projection, the force update and the final
+// boundary update of a step (:145-153) are folded into the next step's RK loads
+// (`corr`, `fold`) and the bulk mean of the forcing is reduced by stage 2 from
+// the predictor; the last step of a run applies them explicitly, so every run
+// ends in the reference's state.
 int finish_step(pll_context* c, bool fused, bool last);
+int finish_deferred(pll_context* c, bool last);
 
 int step_impl(pll_context* c, bool first, bool last) {
   int rc;
@@ -308,11 +312,7 @@ int step_impl(pll_context* c, bool first, bool last) {
     if ((rc = launch_rk(c, 2, 0))) return rc;  // :125
     if ((rc = launch_bc_vel(c, 0))) return rc;  // :129
     if ((rc = launch_calc_rhs(c))) return rc;   // :133
-    if ((rc = launch_fft_x_fwd(c))) return rc;  // :137
-    if ((rc = launch_fft_y_fwd(c))) return rc;
-    if ((rc = launch_trid(c))) return rc;
-    if ((rc = launch_fft_y_bwd(c))) return rc;
-    if ((rc = launch_fft_x_bwd_step(c))) return rc;
+    if ((rc = launch_poisson_step(c, false))) return rc;  // :137
     return finish_step(c, false, last);
   }
   const bool fused = !c->is_forced[1] && !c->is_forced[2];
@@ -329,39 +329,46 @@ int step_impl(pll_context* c, bool first, bool last) {
     if ((rc = launch_rk(c, 1, fold, corr, 0))) return rc;       // :117
     if ((rc = launch_rk(c, 2, fold, corr, acc, 1))) return rc;  // :125
     c->rhs_bcfold = 1;
-    rc = launch_fft_x_fwd_rhs(c);
+    rc = launch_poisson_step(c, true);  // :133-137
     c->rhs_bcfold = 0;
     if (rc) return rc;
-    if ((rc = launch_fft_y_fwd(c))) return rc;
-    if ((rc = launch_trid(c))) return rc;
-    if ((rc = launch_fft_y_bwd(c))) return rc;
-    if ((rc = launch_fft_x_bwd_step(c))) return rc;
-    if ((rc = launch_bc_p(c))) return rc;                    // :141
-    if (!last) return PLL_OK;
-    if ((rc = launch_bc_vel(c, 1))) return rc;                          // :121, u1's halos
-    if ((rc = launch_correct_vel(c, c->is_forced[0] != 0))) return rc;  // :145 (+ mean of :149)
-    if (c->is_forced[0] && (rc = launch_add_force_u(c))) return rc;     // :149
-    return launch_bc_vel(c, 0);                                          // :153
+    return finish_deferred(c, last);
   }
-  if ((rc = launch_rk(c, 1, fold))) return rc;    // src/main.cc:117
-  if (!fused || c->prec != PLL_FP64) {
-    if ((rc = launch_bc_vel(c, 1))) return rc;    // :121
-    if ((rc = launch_rk(c, 2, fold))) return rc;  // :125
-    if ((rc = launch_bc_vel(c, 0))) return rc;    // :129
-    if ((rc = launch_calc_rhs(c))) return rc;     // :133
-  } else {
-    // :121-133 in one kernel: the u1 boundary is rebuilt on load, the middle
-    // boundary update only matters for calcRHS (computed in the same kernel)
-    // and for w(nz) = 0 (written there); the last step refreshes u1's halos.
-    if ((rc = launch_rk2_rhs(c, fold))) return rc;
-    if (last && (rc = launch_bc_vel(c, 1))) return rc;
+  if (fused && c->prec == PLL_FP64) {
+    // FP64: the projection of step n (:145) and its forcing (:149) are applied
+    // by step n+1's loads (corr, fold); :121-133 run in one kernel: the u1
+    // boundary is rebuilt on load, the middle boundary update only matters for
+    // calcRHS (computed in the same kernel) and for w(nz) = 0 (written there).
+    // That kernel sums the predictor's bulk mean for the forcing.
+    const int corr = fold;
+    const int acc = (!last && c->is_forced[0]) ? 1 : 0;
+    if ((rc = launch_rk(c, 1, fold, corr))) return rc;        // :117
+    if ((rc = launch_rk2_rhs(c, fold, corr, acc))) return rc;  // :121-133
+    if ((rc = launch_poisson_step(c, false))) return rc;       // :137
+    return finish_deferred(c, last);
   }
-  if ((rc = launch_fft_x_fwd(c))) return rc;      // :137
-  if ((rc = launch_fft_y_fwd(c))) return rc;
-  if ((rc = launch_trid(c))) return rc;
-  if ((rc = launch_fft_y_bwd(c))) return rc;
-  if ((rc = launch_fft_x_bwd_step(c))) return rc;
+  // unfused, or FP32-mixed without the calcRHS transform
+  if ((rc = launch_rk(c, 1, fold))) return rc;  // src/main.cc:117
+  if ((rc = launch_bc_vel(c, 1))) return rc;    // :121
+  if ((rc = launch_rk(c, 2, fold))) return rc;  // :125
+  if ((rc = launch_bc_vel(c, 0))) return rc;    // :129
+  if ((rc = launch_calc_rhs(c))) return rc;     // :133
+  if ((rc = launch_poisson_step(c, false))) return rc;  // :137
   return finish_step(c, fused, last);
+}
+
+// src/main.cc:141 after the pressure solve of a step whose projection, forcing
+// and final boundary update (:145-153) the next step's loads apply; the last
+// step of a run refreshes u1's halos (its boundary update :121 was not run)
+// and applies them explicitly.
+int finish_deferred(pll_context* c, bool last) {
+  int rc;
+  if ((rc = launch_bc_p(c))) return rc;  // :141
+  if (!last) return PLL_OK;
+  if ((rc = launch_bc_vel(c, 1))) return rc;                          // :121, u1's halos
+  if ((rc = launch_correct_vel(c, c->is_forced[0] != 0))) return rc;  // :145 (+ mean of :149)
+  if (c->is_forced[0] && (rc = launch_add_force_u(c))) return rc;     // :149
+  return launch_bc_vel(c, 0);                                          // :153
 }
 
 // src/main.cc:141-153 after the pressure solve.

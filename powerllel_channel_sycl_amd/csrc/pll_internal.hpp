@@ -97,7 +97,7 @@ struct pll_context {
   hipStream_t stream = nullptr;
 
   void* fld[PLL_NFIELD] = {};
-  void* fld_alt[3] = {};  // spare u, v, w: the fused stage 2 writes here, then the roles swap
+  void* fld_alt[4] = {};  // spare u, v, w, p (FP64): the fused stage 2 writes here, then the roles swap
   void* spec = nullptr;
 
   // host mesh in FP64 (index 0..nz+1)
@@ -209,9 +209,11 @@ int launch_rk(pll_context* c, int stage, int fold, int corr = 0, int acc = 0, in
 int launch_rk_closure(pll_context* c, int stage);
 int rk_setup(pll_context* c);  // per-plane coefficients of the FP32-mixed stencil
 int launch_calc_rhs(pll_context* c);
-// RK stage 2 fused with calcRHS (step sequence only): writes the new u, v, w to
-// the spare buffers and swaps them in; defer adds the previous step's force.
-int launch_rk2_rhs(pll_context* c, int defer);
+// RK stage 2 fused with calcRHS (FP64 step sequence only): writes the new u, v,
+// w and the right-hand side to the spare buffers and swaps them in; defer adds
+// the previous step's force, corr applies its projection on load, acc reduces
+// the bulk mean of the new u and the force of u into d_scalar[0], [3].
+int launch_rk2_rhs(pll_context* c, int defer, int corr = 0, int acc = 0);
 // with_mean: also reduce the bulk mean of the corrected u and store the force
 // of u (vel_force[0] - mean) on the device.
 int launch_correct_vel(pll_context* c, bool with_mean);
@@ -241,5 +243,9 @@ int launch_fft_y_fwd(pll_context* c);
 int launch_fft_y_bwd(pll_context* c);
 int trid_setup(pll_context* c);
 int launch_trid(pll_context* c);
+// The pressure solve of the step (src/mod_poisson_solver.cc:454-510) from p
+// (or, with rhs, from calcRHS computed in the x transform's loads) back to p
+// with its periodic x halos.
+int launch_poisson_step(pll_context* c, bool rhs);
 
 }  // namespace pll

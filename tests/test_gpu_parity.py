@@ -273,12 +273,13 @@ def test_full_step_parity_full_grids(name, steps):
 
 
 @pytest.mark.parametrize("name,precision", [("r3", "fp64"), ("c1", "fp64"), ("r3", "fp32_mixed")])
-def test_multistep_folding_is_bitexact(name, precision):
-    """pll_step(n) folds each step's forcing and final boundary update into the
-    next step's loads; in FP64 it must give the same bits as n single steps.
-    The FP32-mixed step also defers the projection into the next step's loads
-    and takes the forcing's bulk mean from the predictor (the projection leaves
-    every x-row sum unchanged), which moves the result by FP32 round-off only."""
+def test_multistep_folding_matches_single_steps(name, precision):
+    """pll_step(n) folds each step's projection, forcing and final boundary
+    update into the next step's loads and takes the forcing's bulk mean from
+    the predictor (the projection leaves every x-row sum unchanged).  The
+    projected values are bit-identical to the separate projection; the bulk
+    mean is the same sum of other FP64 roundings, so n fused steps differ from
+    n single steps by FP64 round-off (FP32 round-off in FP32-mixed)."""
     _print_case("folding", name, precision)
     o = make_oracle(name)
     u, v, w, p = (o.field() for _ in range(4))
@@ -298,7 +299,8 @@ def test_multistep_folding_is_bitexact(name, precision):
             outs.append([ch.download(f) for f in VEL + ("p",)])
     for a, b, fld in zip(outs[0], outs[1], VEL + ("p",)):
         if precision == "fp64":
-            np.testing.assert_array_equal(a, b, err_msg=fld)
+            # fields of magnitude ~1: a few FP64 units in the last place
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-13, err_msg=fld)
         else:
             # FP32 fields of magnitude ~1: a few units in the last place
             np.testing.assert_allclose(a, b, rtol=0, atol=2e-6, err_msg=fld)
