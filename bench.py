@@ -58,8 +58,12 @@ BYTES_PER_POINT = {
 # Timed windows of --steps steps each; the reported time is their median.
 WINDOWS = 3
 
-# The algorithm's compulsory bytes per point per RK2 step (unfused, FP64): the
-# roofline of the whole step; FP32-mixed halves it.
+# The compulsory bytes per point of each routine of the reference's RK2 step
+# (src/main.cc:116-154, FP64, one launch each): their sum is the roofline of the
+# whole step; FP32-mixed halves it.
+ROUTINE_BYTES_FP64 = {"rk1": 48, "rk2": 72, "calc_rhs": 32, "fft_x_fwd": 16, "fft_y_fwd": 16,
+                      "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "correct_vel": 56,
+                      "mean_partial": 8, "force_add": 16}
 STEP_BYTES_FP64 = 312
 
 # HBM bytes per point per launch measured with rocprofv3 on the 768x384x384
@@ -67,10 +71,10 @@ STEP_BYTES_FP64 = 312
 # profiles/retau550_768x384x384_step_kernels.md holds the runs.  Reported as
 # the roofline entry's `traffic` for that workload only.
 MEASURED_HBM_B_PER_POINT = {
-    "fp64": {"fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 38.7, "fft_y_bwd": 16.0,
-             "fft_x_bwd": 16.0, "correct_vel": 59.9},
-    "fp32_mixed": {"rk1": 27.1, "rk2": 38.9, "fft_x_fwd_rhs": 20.8, "fft_y_fwd": 8.0,
-                   "solve_trid": 22.7, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
+    "fp64": {"rk1": 62.0, "rk2_rhs": 94.4, "fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 21.9,
+             "fft_y_bwd": 16.0, "fft_x_bwd": 16.0, "correct_vel": 59.8},
+    "fp32_mixed": {"rk1": 27.3, "rk2": 39.1, "fft_x_fwd_rhs": 20.8, "fft_y_fwd": 8.0,
+                   "solve_trid": 11.9, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
 }
 
 # MFMA flops per training sample of the closure network (forward 2(16*64 +

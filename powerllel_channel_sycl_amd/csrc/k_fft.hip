@@ -1017,7 +1017,7 @@ int launch_poisson_step(pll_context* c, bool rhs) {
   int rc;
   if ((rc = rhs ? launch_fft_x_fwd_rhs(c) : launch_fft_x_fwd(c))) return rc;
   if ((rc = launch_fft_y_fwd(c))) return rc;
-  if ((rc = launch_trid(c))) return rc;
+  if ((rc = launch_trid_step(c))) return rc;
   if ((rc = launch_fft_y_bwd(c))) return rc;
   return launch_fft_x_bwd_step(c);
 }

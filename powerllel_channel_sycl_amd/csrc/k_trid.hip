@@ -21,6 +21,7 @@
 #include "pll_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace pll {
 
@@ -128,6 +129,165 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
 }
 
 
+// ---------------------------------------------------------------------------
+// The same solve inside the step, partitioned so that the work array makes one
+// HBM pass each way.  A block owns kCw adjacent columns of one row j; each
+// column is cut into S segments of M planes, one thread per segment, and the
+// segment's right-hand side and reciprocal pivots stay in registers:
+//   A. forward elimination of the segment from a zero incoming value, with the
+//      running product g of the negated multipliers (the segment end's
+//      dependence on the incoming value);
+//   exchange: the true segment-end values, sequentially over the S segments
+//      (Y_s = yhat_s + g_s * Y_{s-1}), through LDS;
+//   B. the forward elimination again from the true incoming value (the last
+//      row divided by its pivot, src/mod_poisson_solver.cc:341-346);
+//   C, exchange, D: the same for the back substitution (:348-354) from the top.
+// The segment boundaries are combined with other roundings than the
+// sequential sweep, and the pivots are applied as reciprocals, so the result
+// differs from k_trid's by FP64 round-off (it is not bit-identical).
+constexpr int kSegMax = 16;  // segments per column
+constexpr int kSegPlanes = kSegMax * 24;  // the deepest column the segments hold
+
+// The sequential combination of the S segment-end values, Y_q = hat_q + dep_q *
+// Y_{q-1} (forward, REV false) or from the top (REV true), by the threads of
+// one segment (one per column); the LDS reads are issued four at a time ahead
+// of the dependent chain.
+template <int CW, bool REV>
+__device__ __forceinline__ void combine(const double (*hat)[CW], const double (*dep)[CW],
+                                        double (*inc)[CW], int nseg, int col) {
+  double y = 0.0;
+#pragma unroll
+  for (int q0 = 0; q0 < kSegMax; q0 += 4) {
+    double h[4], d[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = REV ? nseg - 1 - (q0 + u) : q0 + u;
+      if (q0 + u < nseg) {
+        h[u] = hat[q][col];
+        d[u] = dep[q][col];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = REV ? nseg - 1 - (q0 + u) : q0 + u;
+      if (q0 + u < nseg) {
+        y = h[u] + d[u] * y;
+        inc[q][col] = y;
+      }
+    }
+  }
+}
+
+template <typename T, int M, int CW>
+__global__ __launch_bounds__(CW* kSegMax) void k_trid_seg(int nx, int ny, int nz, int nseg,
+                                                        T* __restrict__ var,
+                                                        const double* __restrict__ a,
+                                                        const double* __restrict__ c,
+                                                        const double* __restrict__ rpiv, int cxn,
+                                                        int cyn) {
+  __shared__ double hat[kSegMax][CW];  // segment-end value from a zero incoming value
+  __shared__ double dep[kSegMax][CW];  // its dependence on the incoming value
+  __shared__ double inc[kSegMax][CW];  // the true value at the segment end
+  __shared__ double sa[kSegPlanes], sc[kSegPlanes];
+  const int t = threadIdx.x, col = t % CW, s = t / CW;
+  const int i = blockIdx.x * CW + col, j = blockIdx.y;
+  for (int k = t; k < nz; k += CW * nseg) {
+    sa[k] = a[k];
+    sc[k] = c[k];
+  }
+  const bool on = i < nx;
+  const long nxy = (long)nx * ny;
+  const long P = (long)cxn * cyn;
+  const int k0 = s * M;
+  const int n = on ? min(M, nz - k0) : 0;
+  T* v = var + (long)j * nx + i;
+  const double* rp = rpiv + (long)compact_index(j) * cxn + compact_index(on ? i : 0);
+  // the segment's values (in the field precision: FP32-mixed rounds the
+  // forward-eliminated values to FP32 between the sweeps, as k_trid does) and
+  // the reciprocal pivots of planes k0-1 .. k0+n-1
+  T x[M];
+  double rb[M + 1];
+#pragma unroll
+  for (int m = 0; m < M; ++m)
+    if (m < n) x[m] = v[(long)(k0 + m) * nxy];
+#pragma unroll
+  for (int m = 0; m <= M; ++m)
+    if (m <= n && k0 - 1 + m >= 0) rb[m] = rp[(long)(k0 - 1 + m) * P];
+  __syncthreads();  // sa, sc
+  // A: forward elimination from a zero incoming value
+  {
+    double y = 0.0, g = 1.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+      if (m < n) {
+        const int k = k0 + m;
+        if (k == 0) {
+          y = (double)x[0];
+          g = 0.0;
+        } else {
+          const double l = sa[k] * rb[m];
+          y = (double)x[m] - l * y;
+          g = -l * g;
+        }
+      }
+    if (on) {
+      hat[s][col] = y;
+      dep[s][col] = g;
+    }
+  }
+  __syncthreads();
+  if (s == 0) combine<CW, false>(hat, dep, inc, nseg, col);
+  __syncthreads();
+  // B: forward elimination from the true incoming value; the last row
+  {
+    double y = s > 0 && on ? inc[s - 1][col] : 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+      if (m < n) {
+        const int k = k0 + m;
+        y = k == 0 ? (double)x[0] : (double)x[m] - (sa[k] * rb[m]) * y;
+        if (k == nz - 1) y = y * rb[m + 1];  // the reciprocal table holds 0 for a zero pivot
+        x[m] = (T)y;
+      }
+  }
+  __syncthreads();  // inc is rewritten below
+  // C: back substitution from a zero value above the segment
+  {
+    double xv = 0.0, h = 1.0;
+#pragma unroll
+    for (int m = M - 1; m >= 0; --m)
+      if (m < n) {
+        const int k = k0 + m;
+        if (k == nz - 1) {
+          xv = (double)x[m];
+          h = 0.0;
+        } else {
+          const double q = sc[k] * rb[m + 1];
+          xv = (double)x[m] * rb[m + 1] - q * xv;
+          h = -q * h;
+        }
+      }
+    if (on) {
+      hat[s][col] = xv;
+      dep[s][col] = h;
+    }
+  }
+  __syncthreads();
+  if (s == 0) combine<CW, true>(hat, dep, inc, nseg, col);
+  __syncthreads();
+  // D: back substitution from the true value above the segment
+  {
+    double xv = s + 1 < nseg && on ? inc[s + 1][col] : 0.0;
+#pragma unroll
+    for (int m = M - 1; m >= 0; --m)
+      if (m < n) {
+        const int k = k0 + m;
+        xv = k == nz - 1 ? (double)x[m] : (double)x[m] * rb[m + 1] - (sc[k] * rb[m + 1]) * xv;
+        v[(long)k * nxy] = (T)xv;
+      }
+  }
+}
+
 }  // namespace
 
 // init_poisson_solver's eigenvalue, coefficient and pivot set-up
@@ -177,6 +337,13 @@ int trid_setup(pll_context* c) {
   if (c->cfg.pin_singular || c->prec != PLL_FP64) piv[(size_t)(nz - 1) * P] = 0.0;
   // FP32-mixed multiplies by reciprocal pivots (0 stands for a zero pivot:
   // the reference's b == 0 branch gives 0, src/mod_poisson_solver.cc:341-346)
+  // The partitioned solve of the step multiplies by reciprocal pivots in
+  // both precisions; FP64 keeps the pivots as well for the entry point.
+  std::vector<double> rpiv;
+  if (c->prec == PLL_FP64 && nz <= kSegPlanes) {
+    rpiv.resize(piv.size());
+    for (size_t t = 0; t < piv.size(); ++t) rpiv[t] = piv[t] != 0.0 ? 1.0 / piv[t] : 0.0;
+  }
   if (c->prec != PLL_FP64)
     for (double& x : piv) x = x != 0.0 ? 1.0 / x : 0.0;
 
@@ -186,6 +353,11 @@ int trid_setup(pll_context* c) {
   PLL_HIP_TRY(hipMemcpy(c->d_a, a.data(), nz * sizeof(double), hipMemcpyHostToDevice));
   PLL_HIP_TRY(hipMemcpy(c->d_c, cc.data(), nz * sizeof(double), hipMemcpyHostToDevice));
   PLL_HIP_TRY(hipMemcpy(c->d_piv, piv.data(), piv.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (!rpiv.empty()) {
+    PLL_HIP_TRY(ctx_malloc(c, &c->d_rpiv, rpiv.size() * sizeof(double)));
+    PLL_HIP_TRY(hipMemcpy(c->d_rpiv, rpiv.data(), rpiv.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+  }
   return PLL_OK;
 }
 
@@ -200,6 +372,47 @@ int launch_trid(pll_context* c) {
     hipLaunchKernelGGL((k_trid<float, true>), grid, dim3(kTx, kTy), 0, c->stream, c->g.nx,
                        c->g.ny, c->g.nz, static_cast<float*>(c->spec), c->d_a, c->d_c, c->d_piv,
                        c->cxn, c->cyn);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
+
+namespace {
+// columns per block of the partitioned solve
+template <typename T>
+constexpr int seg_cols() {
+  return std::is_same<T, float>::value ? 32 : 16;
+}
+
+template <typename T, int M>
+void trid_seg(pll_context* c, const double* rpiv) {
+  constexpr int kCw = seg_cols<T>();
+  const int nseg = (c->g.nz + M - 1) / M;
+  const dim3 grid((c->g.nx + kCw - 1) / kCw, c->g.ny);
+  hipLaunchKernelGGL((k_trid_seg<T, M, kCw>), grid, dim3(kCw * nseg), 0, c->stream, c->g.nx,
+                     c->g.ny, c->g.nz, nseg, static_cast<T*>(c->spec), c->d_a, c->d_c, rpiv,
+                     c->cxn, c->cyn);
+}
+template <typename T>
+void trid_seg_t(pll_context* c, const double* rpiv) {
+  if (c->g.nz <= kSegMax * 8)
+    trid_seg<T, 8>(c, rpiv);
+  else if (c->g.nz <= kSegMax * 16)
+    trid_seg<T, 16>(c, rpiv);
+  else
+    trid_seg<T, 24>(c, rpiv);
+}
+}  // namespace
+
+int launch_trid_step(pll_context* c) {
+  // FP32-mixed keeps reciprocal pivots in d_piv; FP64 has them in d_rpiv when
+  // the columns fit the segments (nz <= 384), else the sequential kernel runs
+  const double* rpiv = c->prec == PLL_FP64 ? c->d_rpiv : c->d_piv;
+  if (!rpiv || c->g.nz > kSegPlanes) return launch_trid(c);
+  TimerScope ts(c, K_TRID);
+  if (c->prec == PLL_FP64)
+    trid_seg_t<double>(c, rpiv);
+  else
+    trid_seg_t<float>(c, rpiv);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }

@@ -115,6 +115,7 @@ struct pll_context {
   double* d_a = nullptr;
   double* d_c = nullptr;
   double* d_piv = nullptr;  // FP64 pivots (FP32-mixed: their reciprocals), cxn*cyn*nz
+  double* d_rpiv = nullptr;  // FP64: the reciprocal pivots (partitioned solve of the step)
 
   // FFT plans and twiddles (in field precision)
   pll::FftPlan plan_x;  // complex length nx/2
@@ -243,6 +244,8 @@ int launch_fft_y_fwd(pll_context* c);
 int launch_fft_y_bwd(pll_context* c);
 int trid_setup(pll_context* c);
 int launch_trid(pll_context* c);
+// the solve inside the step: columns partitioned into register-held segments
+int launch_trid_step(pll_context* c);
 // The pressure solve of the step (src/mod_poisson_solver.cc:454-510) from p
 // (or, with rhs, from calcRHS computed in the x transform's loads) back to p
 // with its periodic x halos.

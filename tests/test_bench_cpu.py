@@ -37,17 +37,20 @@ def test_defaults_and_workloads(monkeypatch):
 
 def test_step_bytes_add_up():
     b = _bench()
+    u = b.ROUTINE_BYTES_FP64
+    assert sum(u.values()) == b.STEP_BYTES_FP64 == 312
     k = b.BYTES_PER_POINT["fp64"]
-    unfused = ("rk1", "rk2", "calc_rhs", "fft_x_fwd", "fft_y_fwd", "solve_trid", "fft_y_bwd",
-               "fft_x_bwd", "correct_vel", "mean_partial", "force_add")
-    assert sum(k[n] for n in unfused) == b.STEP_BYTES_FP64 == 312
-    # the fused stage 2 + calcRHS no longer reads u, v, w a second time
-    assert k["rk2_rhs"] == k["rk2"] + k["calc_rhs"] - 24
-    # FP32 fields halve every field access; the FP32-mixed stages also read p
-    # (the previous step's projection applied on load)
+    for n in ("rk2", "calc_rhs", "fft_x_fwd", "fft_y_fwd", "solve_trid", "fft_y_bwd", "fft_x_bwd",
+              "correct_vel", "mean_partial", "force_add"):
+        assert k[n] == u[n], n
+    # inside the step stage 1 also reads p (the previous step's projection is
+    # applied on load); the fused stage 2 + calcRHS reads u, v, w once, and p
+    assert k["rk1"] == u["rk1"] + 8
+    assert k["rk2_rhs"] == u["rk2"] + u["calc_rhs"] - 24 + 8
+    # FP32 fields halve every field access; FP32-mixed stage 2 also reads p
     f = b.BYTES_PER_POINT["fp32_mixed"]
-    for n in ("calc_rhs", "fft_x_fwd", "fft_y_fwd", "solve_trid", "fft_y_bwd", "fft_x_bwd",
+    for n in ("rk1", "calc_rhs", "fft_x_fwd", "fft_y_fwd", "solve_trid", "fft_y_bwd", "fft_x_bwd",
               "correct_vel", "mean_partial", "force_add"):
         assert 2 * f[n] == k[n], n
-    assert f["rk1"] == k["rk1"] // 2 + 4 and f["rk2"] == k["rk2"] // 2 + 4
+    assert f["rk2"] == k["rk2"] // 2 + 4
     assert b.STAT_BYTES == {"fp64": 208, "fp32_mixed": 192}
