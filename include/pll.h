@@ -216,6 +216,16 @@ int pll_closure_set_params(pll_closure *c, const float *host);
 /* Gradient of the mean squared error of softplus(o) against y (host copy), and the loss. */
 int pll_closure_gradient(pll_closure *c, const float *x, const float *y, long n, float *grad_host,
                          double *loss);
+/* pll_closure_gradient without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight training loops).  The kernels' partial gradients, chunk sums
+ * and result live in one static device array of 6.1 MB per GPU (every
+ * training grid fits: at most 1024 waves); the call allocates nothing on the
+ * device or host.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (other streams, other threads, other closures) race.
+ * The closure's device (current at pll_closure_create) must be current. */
+int pll_closure_gradient_noalloc(pll_closure *c, const float *x, const float *y, long n,
+                                 float *grad_host, double *loss);
 /* One Adam step (beta 0.9, 0.999, eps 1e-8) on the full batch. */
 int pll_closure_train_step(pll_closure *c, const float *x, const float *y, long n, double lr);
 int pll_closure_predict(pll_closure *c, const float *x, long n, float *out);

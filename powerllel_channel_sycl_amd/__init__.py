@@ -108,6 +108,8 @@ _SIGNATURES = {
     "pll_closure_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "pll_closure_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
                                        C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+    "pll_closure_gradient_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                               C.POINTER(C.c_float), C.POINTER(C.c_double)]),
     "pll_closure_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
                                          C.c_double]),
     "pll_closure_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]),
@@ -216,6 +218,15 @@ class Closure:
         _check(load_library().pll_closure_gradient(
             self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(n),
             g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)), "pll_closure_gradient")
+        return g, loss.value
+
+    def gradient_noalloc(self, x_ptr, y_ptr, n):
+        """gradient() through pll_closure_gradient_noalloc (no allocation)."""
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_double()
+        _check(load_library().pll_closure_gradient_noalloc(
+            self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(n),
+            g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)), "pll_closure_gradient_noalloc")
         return g, loss.value
 
     def train_step(self, x_ptr, y_ptr, n, lr):

@@ -26,6 +26,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <random>
 #include <vector>
@@ -997,6 +998,7 @@ struct pll_closure {
   float* chunks = nullptr;  // first-level gradient sums, (kMaxWaves / kRedChunk) x (kNParam + 1)
   long step = 0;
   hipStream_t stream = nullptr;
+  int device = 0;              // the device current at creation (that of stream)
   hipEvent_t ready = nullptr;  // orders a context's closure stages after training
   // feature and output scales: inputs g*gs and (wall distance)*ys, nu_t = nus*softplus(o)
   double gs = 1.0, ys = 1.0, nus = 1.0;
@@ -1055,6 +1057,38 @@ int gradient(pll_closure* c, const float* x, const float* y, long n) {
   int rc = PLL_OK;
   gradient_chunks(c, x, y, n, true, &rc);
   return rc;
+}
+
+// ---- allocation-free gradient ------------------------------------------------
+// The buffers of pll_closure_gradient_noalloc live in one file-scope device
+// array, one copy per GPU for the life of the process: the per-block partial
+// gradients of the largest training grid (kMaxWaves / kWaves blocks), the
+// first-level chunk sums and the summed gradient with the loss, each
+// kNParam + 1 floats -- 6.1 MB.  Every call on a GPU shares that GPU's copy;
+// each call writes every element it reads before reading it, so nothing has
+// to start at zero, and calls that overlap on one GPU race.
+constexpr long kScratchPartials = kMaxWaves / kWaves;
+constexpr long kScratchChunks = kMaxWaves / kRedChunk;
+constexpr long kScratchFloats = (kScratchPartials + kScratchChunks + 1) * (kNParam + 1);
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+// Device address of `scratch` on each GPU, resolved once per device with that
+// device current (the host-side name is only a stand-in for the symbol).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on(int dev, float** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(PLL_ERR_STATE, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return PLL_OK;
 }
 
 // Label regularisation: eps of nu* in units of the squared scaled strain rate,
@@ -1326,7 +1360,9 @@ int pll_closure_create(uint64_t seed, pll_closure** out) {
     pll_closure_destroy(c);
     return rc;
   };
-  hipError_t e = hipStreamCreate(&c->stream);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipGetDevice"));
+  e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) return bail(hip_fail(e, "closure stream"));
   e = hipEventCreateWithFlags(&c->ready, hipEventDisableTiming);
   if (e != hipSuccess) return bail(hip_fail(e, "closure event"));
@@ -1389,6 +1425,43 @@ int pll_closure_set_params(pll_closure* c, const float* host) {
   hipLaunchKernelGGL(k_refresh_bf16, dim3((kNParam + 255) / 256), dim3(256), 0, c->stream, c->p,
                      c->wb);
   PLL_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PLL_OK;
+}
+
+int pll_closure_gradient_noalloc(pll_closure* c, const float* x, const float* y, long n,
+                                 float* grad_host, double* loss) {
+  if (!c || !x || !y || n <= 0) return fail(PLL_ERR_ARG, "bad arguments");
+  int dev = -1;
+  PLL_HIP_TRY(hipGetDevice(&dev));
+  if (dev != c->device)
+    return fail(PLL_ERR_STATE, "pll_closure_gradient_noalloc: the closure's device must be current");
+  const long nw = waves_for(n), nblk = nw / kWaves;
+  const long nchunk = (nblk + kRedChunk - 1) / kRedChunk;
+  if (nblk > kScratchPartials || nchunk > kScratchChunks)
+    return fail(PLL_ERR_ARG, "pll_closure_gradient_noalloc: the training grid exceeds the scratch "
+                             "array (at most kMaxWaves waves)");
+  float* base = nullptr;
+  int rc = scratch_on(dev, &base);
+  if (rc) return rc;
+  float* partial = base;
+  float* chunks = base + kScratchPartials * (kNParam + 1);
+  float* grad = chunks + kScratchChunks * (kNParam + 1);
+  // the kernels of pll_closure_gradient, on the scratch array
+  hipLaunchKernelGGL((k_mlp<true>), dim3((unsigned)nblk), dim3(64 * kWaves), 0, c->stream,
+                     weights(c), x, y, n, static_cast<float*>(nullptr), partial,
+                     tiles_per_wave(n, nw));
+  hipLaunchKernelGGL(k_grad_reduce, dim3((kNParam + 1 + 255) / 256, (unsigned)nchunk), dim3(256), 0,
+                     c->stream, partial, nblk, chunks);
+  hipLaunchKernelGGL(k_grad_final, dim3((kNParam + 1 + 255) / 256), dim3(256), 0, c->stream, chunks,
+                     nchunk, grad);
+  PLL_HIP_TRY(hipGetLastError());
+  float lsum = 0.f;
+  if (grad_host)
+    PLL_HIP_TRY(hipMemcpyAsync(grad_host, grad, (size_t)kNParam * sizeof(float),
+                               hipMemcpyDeviceToHost, c->stream));
+  PLL_HIP_TRY(hipMemcpyAsync(&lsum, grad + kNParam, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  PLL_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (loss) *loss = (double)lsum / (double)n;
   return PLL_OK;
 }
 
