@@ -1276,6 +1276,8 @@ int mean_t(pll_context* c, int field, double nxg, double nyg, double* d_out, dou
            double* d_force) {
   const int chunks = c->mean_chunks;
   const int rpc = (c->g.ny + chunks - 1) / chunks;
+  if (chunks < 1 || (long)chunks * c->g.nz > c->npartial || !c->d_partial || !F<T>(c, field) || !d_out)
+    return fail(PLL_ERR_STATE, "bulk-mean reduction scratch does not fit the grid");
   {
     TimerScope ts(c, K_MEAN_PARTIAL);
     hipLaunchKernelGGL((k_mean_partial<T>), dim3(chunks, c->g.nz), dim3(kRedThreads), 0,

@@ -77,6 +77,12 @@ C3_STEPS = int(os.environ.get("PLL_RETAU550_STEPS", "5000")) // 100 * 100
 
 @pytest.mark.timeout(1000)
 def test_fp32_mean_profile_matches_fp64_at_retau550():
+    """FP32-mixed against FP64 from one turbulent state on the whole 768x384x384
+    grid.  With the default window (5000 steps, 10 time units) the two
+    trajectories have not yet decorrelated, so this bounds the FP32 drift of
+    the statistics rather than comparing two independent samples; the
+    decorrelated statistics comparison is the Re_tau~395 test above, and
+    PLL_RETAU550_STEPS=20000 (40 time units) runs it at Re_tau~550."""
     print("CASE fp32 vs fp64 turbulent statistics Re_tau~550 (768x384x384)", flush=True)
     c = C3
     dzf = pll.mesh_by_func(1, 0, 1.8, c["nz"], c["lz"])
