@@ -97,6 +97,10 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of 1 GiB (bandwidth ceiling and the "
                          "scale factor of the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--closure-kernels", action="store_true",
+                    help="run the learned closure's kernels through their own entry points "
+                         "(filter, samples, training, both closure stages; no pll_step) on the "
+                         "closure workflow's grids, print their HIP-event times and exit")
     ap.add_argument("--stat-probe", action="store_true",
                     help="time the run-time statistics accumulation (all 11 sums) on the "
                          "workload grid in both precisions and exit")
@@ -157,6 +161,65 @@ def closure_bench(steps=50, warmup=5):
                        "samples_per_s": round(n / sec), "mfma_tflops": round(n * CLOSURE_TRAIN_FLOP / sec / 1e12, 1)}
     cl.close()
     return out
+
+
+def closure_kernels_probe(reps=5):
+    """The closure's kernels through their entry points, for per-kernel
+    profiles: a fine FP32-mixed 384x256x192 channel filtered onto the coarse
+    FP64 96x64x64 one with the training samples of one snapshot
+    (pll_closure_samples: k_filter, k_samples), Adam steps on them
+    (pll_closure_train_step: k_mlp, k_grad_reduce, k_adam), and the two RK
+    stages of the coarse grid with the closure attached (pll_time_int_vel_rk1
+    / _rk2: k_nut, k_rk_les) with the boundary updates between them."""
+    import torch
+    import numpy as np
+
+    import powerllel_channel_sycl_amd as pll
+    re_b, re_tau = 6875.0, 395.0
+    utau = re_tau / re_b
+    fine = pll.Channel(384, 256, 192, 2 * np.pi, np.pi, 2.0, re_b, 0.004,
+                       pll.mesh_by_func(1, 0, 1.5, 192, 2.0), precision="fp32_mixed")
+    coarse = pll.Channel(96, 64, 64, 2 * np.pi, np.pi, 2.0, re_b, 0.008,
+                         pll.mesh_by_func(1, 0, 1.5, 64, 2.0))
+    cl = pll.Closure(seed=7)
+    cl.set_scales(1.0 / (re_b * utau * utau), utau * re_b / 100.0, 1.0 / re_b)
+    n = 96 * 64 * 64
+    x = torch.empty((n, 10), dtype=torch.float32, device="cuda:0")
+    y = torch.empty((n,), dtype=torch.float32, device="cuda:0")
+    out = {"workload": "closure kernels: fine 384x256x192 FP32-mixed, coarse 96x64x64 FP64"}
+    try:
+        fine.initFlow("log", noise_intensity=0.1)
+        fine.updateBoundVel(0)
+        coarse.timing(True)
+        fine.timing(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            cl.samples(fine, coarse, x.data_ptr(), y.data_ptr())
+        coarse.synchronize()
+        out["samples_us"] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            cl.train_step(x.data_ptr(), y.data_ptr(), n, 1e-3)
+        cl.synchronize()
+        out["adam_step_us"] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+        coarse.updateBoundVel(0)
+        coarse.attach_closure(cl)
+        coarse.timing_reset()
+        for _ in range(reps):
+            coarse.timeIntVelRK1()
+            coarse.updateBoundVel(1)
+            coarse.timeIntVelRK2()
+            coarse.updateBoundVel(0)
+        coarse.synchronize()
+        t = coarse.timing_read()
+        st = t["rk_closure"]
+        out["closure_stage_us"] = round(st[0] / max(st[1], 1) * 1e3, 1)
+        coarse.attach_closure(None)
+    finally:
+        cl.close()
+        coarse.close()
+        fine.close()
+    print(json.dumps(out))
 
 
 def copy_probe():
@@ -329,6 +392,9 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.closure_kernels:
+        closure_kernels_probe()
         return
     if args.stat_probe:
         stat_probe(args.workload)

@@ -600,10 +600,13 @@ __global__ __launch_bounds__(256) void k_samples(
 
 // ---- the closure stage: RK stage (src/mod_calcVel.cc:27-242 in the
 // reference's arithmetic) plus the divergence of the eddy-viscosity stress
-// 2 nu_t S_ij, with nu_t = nus * softplus(MLP(features)) evaluated in the same
-// kernel.  A block owns 32 x 8 x 4 cells: phase 1 runs the network on the
-// 34 x 10 x 6 cells around them (periodic images in x and y; 0 outside
-// 1 <= k <= nz) into LDS, phase 2 updates the velocities.  Staggered stress
+// 2 nu_t S_ij, with nu_t = nus * softplus(MLP(features)).  k_nut evaluates the
+// network once on every cell of the stage's source into nut; the stage kernel
+// k_rk_les owns 32 x 8 x kLz cells (kLz = 1: at the coarse grid's 393,216
+// cells one-plane blocks give 4x the blocks of four-plane ones and run 7 %
+// faster): phase 1 loads nu_t of the 34 x 10 x (kLz + 2) cells
+// around them (periodic images in x and y; 0 outside 1 <= k <= nz) into LDS,
+// phase 2 updates the velocities.  Staggered stress
 // divergence: normal stresses at cell centres, shear stresses at cell edges
 // with nu_t averaged from the four cells around the edge; the wall edges carry
 // no eddy viscosity (the wall stress is the molecular one).
@@ -648,10 +651,25 @@ __device__ __forceinline__ uint32_t pack_relu_bf16(float lo, float hi) {
 struct RegMlp {
   bf16x8 a1[2];      // W1^T rows (hidden units l%32 + 32t), k-slots 8h..8h+7
   bf16x8 a2[2][4];   // W2^T rows (units l%32 + 32t), k-slots of block kb
-  f32x16 c1[2], c2[2];  // biases of the lane's accumulator rows
   uint32_t w3[2][8];    // BF16 W3 of the lane's layer-2 rows j = 2m, 2m+1, packed
   float b3;
 };
+
+// The hidden biases b1 (lb[0..63]) and b2 (lb[64..127]) are read from LDS into
+// the accumulator rows of lane half h, row block t, when a tile starts: the
+// rows j = 4q .. 4q+3 are the consecutive units 8q + 4h + 32t .. +3.
+__device__ __forceinline__ f32x16 bias_rows(const float* lb, int h, int t) {
+  f32x16 c;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 v = *reinterpret_cast<const float4*>(lb + 8 * q + 4 * h + 32 * t);
+    c[4 * q] = v.x;
+    c[4 * q + 1] = v.y;
+    c[4 * q + 2] = v.z;
+    c[4 * q + 3] = v.w;
+  }
+  return c;
+}
 
 __device__ inline void load_reg_mlp(const Mlp& w, int lane, RegMlp& r) {
   const int h = lane / 32, m = lane % 32;
@@ -671,12 +689,6 @@ __device__ inline void load_reg_mlp(const Mlp& w, int lane, RegMlp& r) {
         r.a2[t][kb][e] = (short)w.w2[unit * kH + u];
       }
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int unit = hid_unit(j, h) + 32 * t;
-      r.c1[t][j] = w.p[oB1 + unit];
-      r.c2[t][j] = w.p[oB2 + unit];
-    }
-#pragma unroll
     for (int q = 0; q < 8; ++q)
       r.w3[t][q] = (uint32_t)w.w3[hid_unit(2 * q, h) + 32 * t] |
                    ((uint32_t)w.w3[hid_unit(2 * q + 1, h) + 32 * t] << 16);
@@ -684,15 +696,14 @@ __device__ inline void load_reg_mlp(const Mlp& w, int lane, RegMlp& r) {
   r.b3 = w.p[oB3];
 }
 
-// o of sample lane%32 (in both halves) from the lane's 8 input slots x.  The
-// output dot runs on v_dot2c_f32_bf16 (BF16 h2 times BF16 W3, FP32 sum).
-__device__ inline float reg_forward(const RegMlp& r, const float x[8]) {
-  const bf16x8 b = __builtin_bit_cast(
-      bf16x8, (u32x4){pack_bf16(x[0], x[1]), pack_bf16(x[2], x[3]), pack_bf16(x[4], x[5]),
-                      pack_bf16(x[6], x[7])});
+// o of sample lane%32 (in both halves) from the lane's 8 input slots x; lb
+// holds the hidden biases (bias_rows).  The output dot runs on
+// v_dot2c_f32_bf16 (BF16 h2 times BF16 W3, FP32 sum).
+__device__ inline float reg_forward(const RegMlp& r, const float* lb, int h, const bf16x8 b) {
   f32x16 z1[2];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) z1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.a1[t], b, r.c1[t], 0, 0, 0);
+  for (int t = 0; t < 2; ++t)
+    z1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.a1[t], b, bias_rows(lb, h, t), 0, 0, 0);
   bf16x8 b2[4];
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
@@ -703,7 +714,7 @@ __device__ inline float reg_forward(const RegMlp& r, const float x[8]) {
                         pack_relu_bf16(z[o + 4], z[o + 5]), pack_relu_bf16(z[o + 6], z[o + 7])});
   }
   // the two row blocks' products alternate, so consecutive MFMAs are independent
-  f32x16 z2[2] = {r.c2[0], r.c2[1]};
+  f32x16 z2[2] = {bias_rows(lb + kH, h, 0), bias_rows(lb + kH, h, 1)};
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
@@ -720,10 +731,8 @@ __device__ inline float reg_forward(const RegMlp& r, const float x[8]) {
   return part + __shfl_xor(part, 32) + r.b3;
 }
 
-constexpr int kLx = 32, kLy = 8, kLz = 4;
+constexpr int kLx = 32, kLy = 8, kLz = 1;
 constexpr int kEx = kLx + 2, kEy = kLy + 2, kEz = kLz + 2, kExt = kEx * kEy * kEz;
-constexpr int kLesTiles = (kExt + 31) / 32;
-static_assert(kLesTiles % kWaves == 0, "every wave runs the same number of network tiles");
 static_assert(kLx * kLy == 64 * kWaves, "one thread per (i, j) column of the tile");
 
 // The 16 field values one half of a sample's inputs reads, as (field, offset
@@ -816,77 +825,112 @@ struct LesIn {
   bool valid;
 };
 
+// The eddy viscosity of every cell of a stage's source field, in chunks of
+// 64*kWaves cells per block (grid-stride): first every thread forms the 10
+// features of one cell (all its loads in flight at once) and stores them as
+// the BF16 input k-slots of the network in LDS, then every wave runs the
+// network in registers (reg_forward: the weights loaded once per block) on its
+// 32-cell tiles of the chunk.  The velocity differences and features are formed
+// in FP32 (they are rounded to BF16 next), in cell_gradient's operand order and
+// features()' scaling and reflection signs (du/dz, dv/dz, dw/dx, dw/dy).  nut
+// is interior, x fastest.
+constexpr int kNutChunk = 64 * kWaves;
+template <typename T>
+__global__ __launch_bounds__(64 * kWaves) void k_nut(Geom g, const T* __restrict__ u,
+                                                    const T* __restrict__ v,
+                                                    const T* __restrict__ w, MeshT<T> m,
+                                                    FeatMesh<T> fm, float dxi, float dyi,
+                                                    float qdxi, float qdyi, Mlp net, float gs,
+                                                    float ys, float nus, float* __restrict__ nut,
+                                                    long ncell) {
+  __shared__ __align__(16) float lb[2 * kH];            // b1, b2
+  __shared__ u32x4 xs[2][kNutChunk];                    // k-slots 0..7 and 8..15 per cell
+  for (int q = threadIdx.x; q < 2 * kH; q += kNutChunk) lb[q] = net.p[(q < kH ? oB1 : oB2 - kH) + q];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, half = lane / 32;
+  RegMlp r;
+  load_reg_mlp(net, lane, r);
+  const long nxy = (long)g.nx * g.ny, sx = g.sx, sz = g.sxy;
+  const long nchunk = (ncell + kNutChunk - 1) / kNutChunk;
+  // the 30 field values and 4 plane coefficients of this thread's cell of a
+  // chunk, loaded one chunk ahead of the features that use them
+  const long off[30] = {sx, sx - 1, -sx, -sx - 1, sz, sz - 1, -sz, -sz - 1, 0, -1,           // u
+                        1, 1 - sx, -1, -1 - sx, sz, sz - sx, -sz, -sz - sx, 0, -sx,         // v
+                        1, 1 - sz, -1, -1 - sz, sx, sx - sz, -sx, -sx - sz, 0, -sz};        // w
+  T raw[30];
+  T coef[4];
+  long ecur = 0;
+  auto load = [&](long ch) {
+    const long e0 = ch * kNutChunk + threadIdx.x;
+    ecur = e0;
+    const long e = e0 < ncell ? e0 : ncell - 1;
+    const int i = (int)(e % g.nx) + 1, j = (int)((e / g.nx) % g.ny) + 1, k = (int)(e / nxy) + 1;
+    const long a = g.idx(i, j, k);
+#pragma unroll
+    for (int q = 0; q < 30; ++q) raw[q] = (q < 10 ? u : (q < 20 ? v : w))[a + off[q]];
+    coef[0] = fm.hz[k];
+    coef[1] = m.dzf_inv[k];
+    coef[2] = fm.zs[k];
+    coef[3] = fm.yw[k];
+  };
+  long ch = blockIdx.x;
+  if (ch < nchunk) load(ch);
+  for (; ch < nchunk; ch += gridDim.x) {
+    {
+      float x[30];
+#pragma unroll
+      for (int q = 0; q < 30; ++q) x[q] = (float)raw[q];
+      const float hz = (float)coef[0], dfi = (float)coef[1], zs = (float)coef[2], yw = (float)coef[3];
+      const float qu_y = (x[0] + x[1]) - (x[2] + x[3]), qu_z = (x[4] + x[5]) - (x[6] + x[7]);
+      const float qv_x = (x[10] + x[11]) - (x[12] + x[13]), qv_z = (x[14] + x[15]) - (x[16] + x[17]);
+      const float qw_x = (x[20] + x[21]) - (x[22] + x[23]), qw_y = (x[24] + x[25]) - (x[26] + x[27]);
+      const float du_x = x[8] - x[9], dv_y = x[18] - x[19], dw_z = x[28] - x[29];
+      const bool valid = ecur < ncell;
+      const float f0 = valid ? (du_x * dxi) * gs : 0.f;          // du/dx
+      const float f1 = valid ? (qu_y * qdyi) * gs : 0.f;         // du/dy
+      const float f2 = valid ? ((qu_z * hz) * gs) * zs : 0.f;    // du/dz
+      const float f3 = valid ? (qv_x * qdxi) * gs : 0.f;         // dv/dx
+      const float f4 = valid ? (dv_y * dyi) * gs : 0.f;          // dv/dy
+      const float f5 = valid ? ((qv_z * hz) * gs) * zs : 0.f;    // dv/dz
+      const float f6 = valid ? ((qw_x * qdxi) * gs) * zs : 0.f;  // dw/dx
+      const float f7 = valid ? ((qw_y * qdyi) * gs) * zs : 0.f;  // dw/dy
+      const float f8 = valid ? (dw_z * dfi) * gs : 0.f;          // dw/dz
+      const float f9 = valid ? yw * ys : 0.f;                    // wall distance
+      xs[0][threadIdx.x] = (u32x4){pack_bf16(f0, f1), pack_bf16(f2, f3), pack_bf16(f4, 0.f), 0u};
+      xs[1][threadIdx.x] = (u32x4){pack_bf16(f5, f6), pack_bf16(f7, f8), pack_bf16(f9, 0.f), 0u};
+    }
+    __syncthreads();  // also orders lb before the first tile
+    if (ch + gridDim.x < nchunk) load(ch + gridDim.x);
+#pragma unroll
+    for (int q = 0; q < kNutChunk / 32 / kWaves; ++q) {
+      const int t = wave + q * kWaves;  // tile of the chunk
+      const bf16x8 b = __builtin_bit_cast(bf16x8, xs[half][t * 32 + lane % 32]);
+      const float o = reg_forward(r, lb, half, b);
+      const long e = ch * kNutChunk + t * 32 + lane % 32;
+      if (lane < 32 && e < ncell) nut[e] = softplus(o) * nus;
+    }
+    __syncthreads();  // xs is rewritten by the next chunk
+  }
+}
+
 template <typename T, int STAGE>
 __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
     Geom g, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
-    T* __restrict__ du, T* __restrict__ dv, T* __restrict__ dw, MeshT<T> m, FeatMesh<T> fm,
-    T dxi, T dyi, T qdxi, T qdyi, T rei, T dt, T ucrf, Mlp w, T gs, T ys, float nus,
-    float* __restrict__ nut, int ntx, int nty) {
+    T* __restrict__ du, T* __restrict__ dv, T* __restrict__ dw, MeshT<T> m, T dxi, T dyi,
+    T rei, T dt, T ucrf, const float* __restrict__ nut, int ntx, int nty) {
   __shared__ float nu[kExt];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int b = blockIdx.x;
   const int ti = b % ntx, tj = (b / ntx) % nty, tk = b / (ntx * nty);
   const int i0 = ti * kLx, j0 = tj * kLy, k0 = tk * kLz;  // extended cell (0,0,0) = (i0,j0,k0)
-  // phase 1: eddy viscosity of the extended block, every lane of a wave on
-  // one of 32 samples (half of its inputs), the network in registers; the
-  // field values of a wave's next tile are loaded before its current tile's
-  // network runs
-  {
-    RegMlp r;
-    load_reg_mlp(w, lane, r);
-    const int half = lane / 32;
-    const T* fp[16];
-    long fo[16];
-    les_half_loads(su, sv, sw, g.sx, g.sxy, half, fp, fo);
-    LesIn<T> in;
-    auto fetch = [&](int t) {
-      const int e0 = t * 32 + lane % 32;
-      const int e = e0 < kExt ? e0 : kExt - 1;
-      const int ex = e % kEx, ey = (e / kEx) % kEy, ez = e / (kEx * kEy);
-      int i = i0 + ex, j = j0 + ey, k = k0 + ez;
-      in.valid = e0 < kExt && i <= g.nx + 1 && j <= g.ny + 1 && k >= 1 && k <= g.nz;
-      // periodic images in x and y; a sample outside the domain reads some
-      // interior cell instead and is discarded
-      i = i == 0 ? g.nx : (i > g.nx ? 1 : i);
-      j = j == 0 ? g.ny : (j > g.ny ? 1 : j);
-      k = k < 1 ? 1 : (k > g.nz ? g.nz : k);
-      const long a = g.idx(i, j, k);
-#pragma unroll
-      for (int s = 0; s < 16; ++s) in.v[s] = fp[s][a + fo[s]];
-      in.hz = fm.hz[k];
-      in.dfi = m.dzf_inv[k];
-      in.zs = fm.zs[k];
-      in.yw = fm.yw[k];
-    };
-    fetch(wave);
-    for (int t = wave; t < kLesTiles; t += kWaves) {
-      // cell_gradient's differences and features()' scaling and reflection
-      // signs (du/dz, dv/dz, dw/dx, dw/dy); multiplying by 1 is exact
-      const T one = T(1);
-      const T q0 = ((in.v[0] + in.v[1]) - (in.v[2] + in.v[3])) * (half ? in.hz : qdyi);
-      const T q1 = ((in.v[4] + in.v[5]) - (in.v[6] + in.v[7])) * (half ? qdxi : in.hz);
-      const T q2 = ((in.v[8] + in.v[9]) - (in.v[10] + in.v[11])) * (half ? qdyi : qdxi);
-      const T d0 = (in.v[12] - in.v[13]) * (half ? in.dfi : dxi);
-      const T d1 = (in.v[14] - in.v[15]) * dyi;
-      const float fq0 = (float)((q0 * gs) * (half ? in.zs : one));
-      const float fq1 = (float)((q1 * gs) * in.zs);
-      const float fq2 = (float)((q2 * gs) * (half ? in.zs : one));
-      const float fd0 = (float)(d0 * gs);
-      const float fd1 = half ? (float)(in.yw * ys) : (float)(d1 * gs);
-      const bool valid = in.valid;
-      // input slots: half 0 (du/dx, du/dy, du/dz, dv/dx, dv/dy), half 1
-      // (dv/dz, dw/dx, dw/dy, dw/dz, wall distance)
-      float x[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      x[0] = valid ? (half ? fq0 : fd0) : 0.f;
-      x[1] = valid ? (half ? fq1 : fq0) : 0.f;
-      x[2] = valid ? (half ? fq2 : fq1) : 0.f;
-      x[3] = valid ? (half ? fd0 : fq2) : 0.f;
-      x[4] = valid ? fd1 : 0.f;
-      if (t + kWaves < kLesTiles) fetch(t + kWaves);
-      const int e = t * 32 + lane % 32;
-      const float o = reg_forward(r, x);
-      if (lane < 32 && e < kExt) nu[e] = valid ? softplus(o) * nus : 0.f;
-    }
+  // phase 1: the eddy viscosity (k_nut) of the extended block into LDS: x/y
+  // halo cells from their periodic images, 0 outside the walls
+  for (int e = threadIdx.x; e < kExt; e += 64 * kWaves) {
+    const int ex = e % kEx, ey = (e / kEx) % kEy, ez = e / (kEx * kEy);
+    int i = i0 + ex, j = j0 + ey;
+    const int k = k0 + ez;
+    const bool valid = i <= g.nx + 1 && j <= g.ny + 1 && k >= 1 && k <= g.nz;
+    i = i == 0 ? g.nx : (i > g.nx ? 1 : i);
+    j = j == 0 ? g.ny : (j > g.ny ? 1 : j);
+    nu[e] = valid ? nut[((long)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] : 0.f;
   }
   __syncthreads();
   // phase 2: the stage update of the block's cells, a column of kLz planes
@@ -976,7 +1020,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
         dv[a] = (V(0, 0, 0) + dt * tv + cur.o[1] + dt * s_v) * T(0.5);
         dw[a] = (W(0, 0, 0) + dt * tw + cur.o[2] + dt * s_w) * T(0.5);
       }
-      nut[((long)(k - 1) * g.ny + (j - 1)) * g.nx + (i - 1)] = nu[c];
     }
     if (kz + 1 < kLz) cur = nxt;
   }
@@ -1144,21 +1187,28 @@ int rk_closure_t(pll_context* c, int stage) {
   const int ntx = (g.nx + kLx - 1) / kLx, nty = (g.ny + kLy - 1) / kLy, ntz = (g.nz + kLz - 1) / kLz;
   const dim3 grid((unsigned)ntx * nty * ntz), block(64 * kWaves);
   const T dxi = (T)c->dx_inv, dyi = (T)c->dy_inv;
-  const T qdxi = (T)(0.25 * c->dx_inv), qdyi = (T)(0.25 * c->dy_inv);
   const T rei = (T)c->re_inv, dt = (T)c->dt, ucrf = (T)c->u_crf;
-  const T gs = (T)cl->gs, ys = (T)cl->ys;
   const float nus = (float)cl->nus;
+  const T* su = fld<T>(c, stage == 1 ? PLL_U : PLL_U1);
+  const T* sv = fld<T>(c, stage == 1 ? PLL_V : PLL_V1);
+  const T* sw = fld<T>(c, stage == 1 ? PLL_W : PLL_W1);
+  // the network on every cell of the stage's source (one wave per 32-cell
+  // tile, up to kMaxWaves waves), then the stage with that eddy viscosity
+  const long ncell = c->nspec;
+  const long nblk = std::min((ncell + kNutChunk - 1) / kNutChunk, 512L);
   TimerScope ts(c, K_RK_CLOSURE);
+  hipLaunchKernelGGL((k_nut<T>), dim3((unsigned)nblk), block, 0, c->stream, g,
+                     su, sv, sw, mesh_t<T>(c), feat_mesh<T>(c), (float)c->dx_inv, (float)c->dy_inv,
+                     (float)(0.25 * c->dx_inv), (float)(0.25 * c->dy_inv), weights(cl), (float)cl->gs,
+                     (float)cl->ys, nus, c->d_nut, ncell);
   if (stage == 1)
-    hipLaunchKernelGGL((k_rk_les<T, 1>), grid, block, 0, c->stream, g, fld<T>(c, PLL_U),
-                       fld<T>(c, PLL_V), fld<T>(c, PLL_W), fld<T>(c, PLL_U1), fld<T>(c, PLL_V1),
-                       fld<T>(c, PLL_W1), mesh_t<T>(c), feat_mesh<T>(c), dxi, dyi, qdxi, qdyi, rei,
-                       dt, ucrf, weights(cl), gs, ys, nus, c->d_nut, ntx, nty);
+    hipLaunchKernelGGL((k_rk_les<T, 1>), grid, block, 0, c->stream, g, su, sv, sw,
+                       fld<T>(c, PLL_U1), fld<T>(c, PLL_V1), fld<T>(c, PLL_W1), mesh_t<T>(c), dxi,
+                       dyi, rei, dt, ucrf, c->d_nut, ntx, nty);
   else
-    hipLaunchKernelGGL((k_rk_les<T, 2>), grid, block, 0, c->stream, g, fld<T>(c, PLL_U1),
-                       fld<T>(c, PLL_V1), fld<T>(c, PLL_W1), fld<T>(c, PLL_U), fld<T>(c, PLL_V),
-                       fld<T>(c, PLL_W), mesh_t<T>(c), feat_mesh<T>(c), dxi, dyi, qdxi, qdyi, rei,
-                       dt, ucrf, weights(cl), gs, ys, nus, c->d_nut, ntx, nty);
+    hipLaunchKernelGGL((k_rk_les<T, 2>), grid, block, 0, c->stream, g, su, sv, sw,
+                       fld<T>(c, PLL_U), fld<T>(c, PLL_V), fld<T>(c, PLL_W), mesh_t<T>(c), dxi, dyi,
+                       rei, dt, ucrf, c->d_nut, ntx, nty);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }
