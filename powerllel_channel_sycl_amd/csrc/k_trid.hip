@@ -145,19 +145,18 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
 // The segment boundaries are combined with other roundings than the
 // sequential sweep, and the pivots are applied as reciprocals, so the result
 // differs from k_trid's by FP64 round-off (it is not bit-identical).
-constexpr int kSegMax = 16;  // segments per column
-constexpr int kSegPlanes = kSegMax * 24;  // the deepest column the segments hold
+constexpr int kSegPlanes = 384;  // the deepest column the segments hold
 
 // The sequential combination of the S segment-end values, Y_q = hat_q + dep_q *
 // Y_{q-1} (forward, REV false) or from the top (REV true), by the threads of
 // one segment (one per column); the LDS reads are issued four at a time ahead
 // of the dependent chain.
-template <int CW, bool REV>
+template <int CW, int S, bool REV>
 __device__ __forceinline__ void combine(const double (*hat)[CW], const double (*dep)[CW],
                                         double (*inc)[CW], int nseg, int col) {
   double y = 0.0;
 #pragma unroll
-  for (int q0 = 0; q0 < kSegMax; q0 += 4) {
+  for (int q0 = 0; q0 < S; q0 += 4) {
     double h[4], d[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -178,16 +177,16 @@ __device__ __forceinline__ void combine(const double (*hat)[CW], const double (*
   }
 }
 
-template <typename T, int M, int CW>
-__global__ __launch_bounds__(CW* kSegMax) void k_trid_seg(int nx, int ny, int nz, int nseg,
+template <typename T, int M, int CW, int S>
+__global__ __launch_bounds__(CW* S) void k_trid_seg(int nx, int ny, int nz, int nseg,
                                                         T* __restrict__ var,
                                                         const double* __restrict__ a,
                                                         const double* __restrict__ c,
                                                         const double* __restrict__ rpiv, int cxn,
                                                         int cyn) {
-  __shared__ double hat[kSegMax][CW];  // segment-end value from a zero incoming value
-  __shared__ double dep[kSegMax][CW];  // its dependence on the incoming value
-  __shared__ double inc[kSegMax][CW];  // the true value at the segment end
+  __shared__ double hat[S][CW];  // segment-end value from a zero incoming value
+  __shared__ double dep[S][CW];  // its dependence on the incoming value
+  __shared__ double inc[S][CW];  // the true value at the segment end
   __shared__ double sa[kSegPlanes], sc[kSegPlanes];
   const int t = threadIdx.x, col = t % CW, s = t / CW;
   const int i = blockIdx.x * CW + col, j = blockIdx.y;
@@ -236,7 +235,7 @@ __global__ __launch_bounds__(CW* kSegMax) void k_trid_seg(int nx, int ny, int nz
     }
   }
   __syncthreads();
-  if (s == 0) combine<CW, false>(hat, dep, inc, nseg, col);
+  if (s == 0) combine<CW, S, false>(hat, dep, inc, nseg, col);
   __syncthreads();
   // B: forward elimination from the true incoming value; the last row
   {
@@ -273,7 +272,7 @@ __global__ __launch_bounds__(CW* kSegMax) void k_trid_seg(int nx, int ny, int nz
     }
   }
   __syncthreads();
-  if (s == 0) combine<CW, true>(hat, dep, inc, nseg, col);
+  if (s == 0) combine<CW, S, true>(hat, dep, inc, nseg, col);
   __syncthreads();
   // D: back substitution from the true value above the segment
   {
@@ -377,29 +376,32 @@ int launch_trid(pll_context* c) {
 }
 
 namespace {
-// columns per block of the partitioned solve
+// Columns per block, planes per segment and segments per column of the
+// partitioned solve.
 template <typename T>
 constexpr int seg_cols() {
   return std::is_same<T, float>::value ? 32 : 16;
 }
+constexpr int kSegM = 24, kSegS = 16;
+static_assert(kSegM * kSegS >= kSegPlanes, "the segments hold the deepest column");
 
 template <typename T, int M>
 void trid_seg(pll_context* c, const double* rpiv) {
   constexpr int kCw = seg_cols<T>();
   const int nseg = (c->g.nz + M - 1) / M;
   const dim3 grid((c->g.nx + kCw - 1) / kCw, c->g.ny);
-  hipLaunchKernelGGL((k_trid_seg<T, M, kCw>), grid, dim3(kCw * nseg), 0, c->stream, c->g.nx,
+  hipLaunchKernelGGL((k_trid_seg<T, M, kCw, kSegS>), grid, dim3(kCw * nseg), 0, c->stream, c->g.nx,
                      c->g.ny, c->g.nz, nseg, static_cast<T*>(c->spec), c->d_a, c->d_c, rpiv,
                      c->cxn, c->cyn);
 }
 template <typename T>
 void trid_seg_t(pll_context* c, const double* rpiv) {
-  if (c->g.nz <= kSegMax * 8)
-    trid_seg<T, 8>(c, rpiv);
-  else if (c->g.nz <= kSegMax * 16)
-    trid_seg<T, 16>(c, rpiv);
+  if (c->g.nz <= kSegS * (kSegM / 3))
+    trid_seg<T, kSegM / 3>(c, rpiv);
+  else if (c->g.nz <= kSegS * (2 * kSegM / 3))
+    trid_seg<T, 2 * kSegM / 3>(c, rpiv);
   else
-    trid_seg<T, 24>(c, rpiv);
+    trid_seg<T, kSegM>(c, rpiv);
 }
 }  // namespace
 
