@@ -284,6 +284,14 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
     }
   };
   Point qc{};
+  // FP64: the lower z-face terms of the thread's cell, carried up the column
+  ZFace<T> zf{};
+  if (!kF32 && active) {
+    const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
+    const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
+    const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
+    zf = zface_lower<T>(U, V, W, m, k0);
+  }
   for (int k = k0; k <= k1; ++k) {
     const bool more = k + 2 <= k1 + 1;
     if (more) fetch(k + 2);
@@ -313,7 +321,10 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
       if constexpr (std::is_same<T, double>::value) {
         // reference arithmetic, src/mod_calcVel.cc:81,103,125 and :194,216,238
         T tu, tv, tw;
-        tendency<T>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw);
+        if (crf)
+          tendency_z<T, true, true>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
+        else
+          tendency_z<T, false, true>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
         if (STAGE == 1) {
           du[a] = U(0, 0, 0) + dt * tu;
           dv[a] = V(0, 0, 0) + dt * tv;
@@ -480,16 +491,23 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     if constexpr (PROJ) q = q - dpy(s0, SX + cx) * pdy;
     return fv ? q + f_v : q;
   };
-  // the three stage-2 updates at tile cell (cx, cy) of plane kk; `which` selects components
+  // the three stage-2 updates at tile cell (cx, cy) of plane kk.  FP64: zf
+  // receives the upper z-face terms; with `carry` it also supplies the lower
+  // ones (the thread's own column, marching up)
   auto update = [&](int sm, int s0, int sp, int cx, int cy, int kk, T& nu, T& nv, T& nw,
-                    T ou, T ov, T ow) {
+                    T ou, T ov, T ow, ZFace<T>& zf, bool carry) {
     const int c = cy * SX + cx;
     const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
     const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
     const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
     if constexpr (std::is_same<T, double>::value) {
       T tu, tv, tw;
-      tendency<T>(U, V, W, m, kk, dxi, dyi, rei, ucrf, tu, tv, tw);
+      if (!carry)
+        tendency_z<T, true, false>(U, V, W, m, kk, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
+      else if (crf)
+        tendency_z<T, true, true>(U, V, W, m, kk, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
+      else
+        tendency_z<T, false, true>(U, V, W, m, kk, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
       nu = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
       nv = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
       nw = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
@@ -511,6 +529,7 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
   }
   const int cxi = tx + 2, cyi = ty + 2;  // tile cell of the thread's interior point
   double msum = 0.0;  // FP64 partial of the bulk mean (acc)
+  ZFace<T> zf{}, zx{};  // the column's carried z-face terms; scratch of the extra cells
   int sm = 0, s0 = 1, sp = 2;
   T wprev = T(0);  // w_new(i, j, k-1); 0 below the wall (w(0) = 0)
   if (k0 > 1) {
@@ -528,7 +547,7 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
       // same association as k_correct: ((p(k+1) - p(k)) * dt) * dzc_inv(k)
       if constexpr (PROJ) ow = ow - dpz(1, 2, cyi * SX + cxi) * dt * m.dzc_inv[k0 - 1];
       if (fw) ow = (k0 - 1 == g.nz) ? T(0) : ow + f_w;
-      update(0, 1, 2, cxi, cyi, k0 - 1, nu, nv, nw, T(0), T(0), ow);
+      update(0, 1, 2, cxi, cyi, k0 - 1, nu, nv, nw, T(0), T(0), ow, zf, false);
       wprev = (k0 - 1 == g.nz) ? T(0) : nw;
     }
     __syncthreads();
@@ -546,6 +565,13 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     fetch(k0 + 1);
     commit(2);
     __syncthreads();
+    if (active) {
+      const int c = cyi * SX + cxi;
+      const PlaneAcc<T, SX> U{sh[0][0], sh[0][1], sh[0][2], c};
+      const PlaneAcc<T, SX> V{sh[1][0], sh[1][1], sh[1][2], c};
+      const PlaneAcc<T, SX> W{sh[2][0], sh[2][1], sh[2][2], c};
+      zf = zface_lower<T>(U, V, W, m, k0);
+    }
   }
   for (int k = k0; k <= k1; ++k) {
     const bool more = k + 2 <= k1 + 1;
@@ -565,7 +591,7 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
       if (fv) ov = ov + f_v;
       if (fw) ow = (k == g.nz) ? T(0) : ow + f_w;
       T nu, nv;
-      update(sm, s0, sp, cxi, cyi, k, nu, nv, nw, ou, ov, ow);
+      update(sm, s0, sp, cxi, cyi, k, nu, nv, nw, ou, ov, ow, zf, true);
       if (k == g.nz) nw = T(0);  // src/mod_updateBound.cc:137 (w(nz) = 0)
       un[ty][tx + 1] = nu;
       vn[ty + 1][tx] = nv;
@@ -577,12 +603,12 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     // backward neighbours: v_new on row j0 (wave 0), u_new on column i0 (wave 1)
     if (ty == 0 && i <= g.nx) {
       T nu, nv, nw2;
-      update(sm, s0, sp, cxi, 1, k, nu, nv, nw2, T(0), old_v(s0, cxi, i, j0, k), T(0));
+      update(sm, s0, sp, cxi, 1, k, nu, nv, nw2, T(0), old_v(s0, cxi, i, j0, k), T(0), zx, false);
       vn[0][tx] = nv;
     } else if (ty == 1 && tx < TY && j0 + tx + 1 <= g.ny) {
       T nu, nv, nw2;
       update(sm, s0, sp, 1, tx + 2, k, nu, nv, nw2, old_u(s0, tx + 2, i0, j0 + tx + 1, k), T(0),
-             T(0));
+             T(0), zx, false);
       un[tx][0] = nu;
     }
     __syncthreads();
