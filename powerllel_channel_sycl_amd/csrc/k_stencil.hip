@@ -53,21 +53,45 @@ struct RkUniform {
   float cx, cy, vx, vy, crf;  // S*0.25*dxi, S*0.25*dyi, S*re_inv*dxi^2, S*re_inv*dyi^2, S*0.5*u_crf*dxi
 };
 
-template <bool CRF, class A>
+// The lower z-face convective products of the three components (ab*wb,
+// wb*ab, ab*ab): the upper-face products of the cell below up to FP32
+// rounding (the face interpolation coefficients of plane k-1's upper face are
+// plane k's lower ones), carried up a column like the FP64 ZFace.
+struct ZConv {
+  float u, v, w;
+};
+
+template <class A>
+__device__ __forceinline__ ZConv zconv_lower(const A& U, const A& V, const A& W, const RkCoef& q) {
+#pragma clang fp contract(fast)
+  ZConv z;
+  const float ub = U(0, 0, 0) * q.b1 + U(0, 0, -1) * q.b2;
+  z.u = ub * (W(0, 0, -1) + W(1, 0, -1));
+  const float vb = V(0, 0, 0) * q.b1 + V(0, 0, -1) * q.b2;
+  z.v = (W(0, 0, -1) + W(0, 1, -1)) * vb;
+  const float wb = W(0, 0, 0) + W(0, 0, -1);
+  z.w = wb * wb;
+  return z;
+}
+
+// CARRY: the lower-face products come from zc, which then receives the upper ones.
+template <bool CRF, bool CARRY = false, class A>
 __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W, const RkCoef& q,
                                               const RkUniform& cu, float& iu, float& iv,
-                                              float& iw) {
+                                              float& iw, ZConv* zc = nullptr) {
   // the FP32 form is not held to the reference's rounding: let products fuse into FMAs
 #pragma clang fp contract(fast)
   {
     const float q0 = U(0, 0, 0), qe = U(1, 0, 0), qw = U(-1, 0, 0), qn = U(0, 1, 0);
     const float qs = U(0, -1, 0), qt = U(0, 0, 1), qb = U(0, 0, -1);
     const float ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
-    const float at = q0 * q.a1 + qt * q.a2, ab = q0 * q.b1 + qb * q.b2;
+    const float at = q0 * q.a1 + qt * q.a2;
     const float vn = V(0, 0, 0) + V(1, 0, 0), vs = V(0, -1, 0) + V(1, -1, 0);
-    const float wt = W(0, 0, 0) + W(1, 0, 0), wb = W(0, 0, -1) + W(1, 0, -1);
-    float conv = cu.cx * (ae * ae - aw * aw) + cu.cy * (an * vn - as * vs) +
-                 q.zuv * (at * wt - ab * wb);
+    const float wt = W(0, 0, 0) + W(1, 0, 0);
+    const float ct = at * wt;
+    const float cb = CARRY ? zc->u : (q0 * q.b1 + qb * q.b2) * (W(0, 0, -1) + W(1, 0, -1));
+    if (CARRY) zc->u = ct;
+    float conv = cu.cx * (ae * ae - aw * aw) + cu.cy * (an * vn - as * vs) + q.zuv * (ct - cb);
     if (CRF) conv += cu.crf * (ae - aw);
     // viscous part as neighbour sums minus the diagonal: fewer operations
     iu = cu.vx * (qe + qw) + cu.vy * (qn + qs) + q.cp * qt + q.cm * qb - q.cdu * q0 - conv;
@@ -76,11 +100,13 @@ __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W
     const float q0 = V(0, 0, 0), qe = V(1, 0, 0), qw = V(-1, 0, 0), qn = V(0, 1, 0);
     const float qs = V(0, -1, 0), qt = V(0, 0, 1), qb = V(0, 0, -1);
     const float ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
-    const float at = q0 * q.a1 + qt * q.a2, ab = q0 * q.b1 + qb * q.b2;
+    const float at = q0 * q.a1 + qt * q.a2;
     const float ue = U(0, 0, 0) + U(0, 1, 0), uw = U(-1, 0, 0) + U(-1, 1, 0);
-    const float wt = W(0, 0, 0) + W(0, 1, 0), wb = W(0, 0, -1) + W(0, 1, -1);
-    float conv = cu.cx * (ue * ae - uw * aw) + cu.cy * (an * an - as * as) +
-                 q.zuv * (wt * at - wb * ab);
+    const float wt = W(0, 0, 0) + W(0, 1, 0);
+    const float ct = wt * at;
+    const float cb = CARRY ? zc->v : (W(0, 0, -1) + W(0, 1, -1)) * (q0 * q.b1 + qb * q.b2);
+    if (CARRY) zc->v = ct;
+    float conv = cu.cx * (ue * ae - uw * aw) + cu.cy * (an * an - as * as) + q.zuv * (ct - cb);
     if (CRF) conv += cu.crf * (ae - aw);
     iv = cu.vx * (qe + qw) + cu.vy * (qn + qs) + q.cp * qt + q.cm * qb - q.cdu * q0 - conv;
   }
@@ -88,13 +114,21 @@ __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W
     const float q0 = W(0, 0, 0), qe = W(1, 0, 0), qw = W(-1, 0, 0), qn = W(0, 1, 0);
     const float qs = W(0, -1, 0), qt = W(0, 0, 1), qb = W(0, 0, -1);
     const float ae = q0 + qe, aw = qw + q0, an = q0 + qn, as = qs + q0;
-    const float at = q0 + qt, ab = q0 + qb;
+    const float at = q0 + qt;
     const float ue = U(0, 0, 0) * q.a1 + U(0, 0, 1) * q.a2;
     const float uw = U(-1, 0, 0) * q.a1 + U(-1, 0, 1) * q.a2;
     const float vn = V(0, 0, 0) * q.a1 + V(0, 0, 1) * q.a2;
     const float vs = V(0, -1, 0) * q.a1 + V(0, -1, 1) * q.a2;
-    float conv = cu.cx * (ue * ae - uw * aw) + cu.cy * (vn * an - vs * as) +
-                 q.zw * (at * at - ab * ab);
+    const float ct = at * at;
+    float cb;
+    if (CARRY) {
+      cb = zc->w;
+      zc->w = ct;
+    } else {
+      const float ab = q0 + qb;
+      cb = ab * ab;
+    }
+    float conv = cu.cx * (ue * ae - uw * aw) + cu.cy * (vn * an - vs * as) + q.zw * (ct - cb);
     if (CRF) conv += cu.crf * (ae - aw);
     iw = cu.vx * (qe + qw) + cu.vy * (qn + qs) + q.dp * qt + q.dm * qb - q.cdw * q0 - conv;
   }
@@ -286,11 +320,15 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
   Point qc{};
   // FP64: the lower z-face terms of the thread's cell, carried up the column
   ZFace<T> zf{};
-  if (!kF32 && active) {
+  ZConv zc{};  // FP32: the carried lower-face convective products
+  if (active) {
     const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
     const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
     const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
-    zf = zface_lower<T>(U, V, W, m, k0);
+    if constexpr (kF32)
+      zc = zconv_lower(U, V, W, coef[k0]);
+    else
+      zf = zface_lower<T>(U, V, W, m, k0);
   }
   for (int k = k0; k <= k1; ++k) {
     const bool more = k + 2 <= k1 + 1;
@@ -340,9 +378,9 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
         const RkCoef q = coef[k];
         float iu, iv, iw;
         if (crf)
-          tendency_fast<true>(U, V, W, q, cu, iu, iv, iw);
+          tendency_fast<true, true>(U, V, W, q, cu, iu, iv, iw, &zc);
         else
-          tendency_fast<false>(U, V, W, q, cu, iu, iv, iw);
+          tendency_fast<false, true>(U, V, W, q, cu, iu, iv, iw, &zc);
         if (STAGE == 1) {
           du[a] = U(0, 0, 0) + iu;
           dv[a] = V(0, 0, 0) + iv;
