@@ -45,13 +45,17 @@ DEFAULT_WORKLOAD = "retau550_768x384x384"
 # solve read and write the work array once; correctVel (last step of a run)
 # reads p,u,v,w and writes u,v,w (the bulk mean of u rides along); the separate
 # bulk mean reads u; the force update reads and writes u.  FP32-mixed (4-byte
-# fields): stage 2 reads u1,v1,w1,u,v,w,p and writes u,v,w, and calcRHS runs
-# inside the forward x transform (reads u,v,w, writes the work array).
+# fields): inside a run both stages run in one kernel (rk12: reads u,v,w,p,
+# writes u,v,w; u1 stays on chip); the first and last step of a run take the
+# separate stages, where stage 2 reads u1,v1,w1,u,v,w,p and writes u,v,w.
+# calcRHS runs inside the forward x transform (reads u,v,w, writes the work
+# array).
 BYTES_PER_POINT = {
     "fp64": {"rk1": 56, "rk2": 72, "calc_rhs": 32, "rk2_rhs": 88, "fft_x_fwd": 16,
              "fft_y_fwd": 16, "solve_trid": 16, "fft_y_bwd": 16, "fft_x_bwd": 16,
              "correct_vel": 56, "mean_partial": 8, "force_add": 16},
-    "fp32_mixed": {"rk1": 28, "rk2": 40, "calc_rhs": 16, "fft_x_fwd": 8, "fft_x_fwd_rhs": 16,
+    "fp32_mixed": {"rk1": 28, "rk2": 40, "rk12": 28, "calc_rhs": 16, "fft_x_fwd": 8,
+                   "fft_x_fwd_rhs": 16,
                    "fft_y_fwd": 8, "solve_trid": 8, "fft_y_bwd": 8, "fft_x_bwd": 8,
                    "correct_vel": 28, "mean_partial": 4, "force_add": 8},
 }

@@ -74,6 +74,30 @@ __device__ __forceinline__ ZConv zconv_lower(const A& U, const A& V, const A& W,
   return z;
 }
 
+// The upper z-face convective products of the cell at the accessor's plane
+// (at*wt, wt*at, at*at), the expressions of tendency_fast: what its CARRY form
+// hands to the plane above.
+template <class A>
+__device__ __forceinline__ ZConv zconv_upper(const A& U, const A& V, const A& W, const RkCoef& q) {
+#pragma clang fp contract(fast)
+  ZConv z;
+  {
+    const float at = U(0, 0, 0) * q.a1 + U(0, 0, 1) * q.a2;
+    const float wt = W(0, 0, 0) + W(1, 0, 0);
+    z.u = at * wt;
+  }
+  {
+    const float at = V(0, 0, 0) * q.a1 + V(0, 0, 1) * q.a2;
+    const float wt = W(0, 0, 0) + W(0, 1, 0);
+    z.v = wt * at;
+  }
+  {
+    const float at = W(0, 0, 0) + W(0, 0, 1);
+    z.w = at * at;
+  }
+  return z;
+}
+
 // CARRY: the lower-face products come from zc, which then receives the upper ones.
 template <bool CRF, bool CARRY = false, class A>
 __device__ __forceinline__ void tendency_fast(const A& U, const A& V, const A& W, const RkCoef& q,
@@ -413,6 +437,245 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
       __syncthreads();
     }
     if (threadIdx.x == 0) partial[b] = red[0];
+  }
+}
+
+// Both RK stages of an FP32-mixed step inside a run (src/mod_calcVel.cc:61-125
+// then :174-238) in one pass.  The stored velocity is the previous step's
+// predictor u*; its projection, forcing and boundary update are applied on
+// load as in k_rk's stage 1 (fold and corr).  A block owns a 64 x 8 tile of
+// stage-2 points and marches a chunk of k planes.  Stage 1 runs on the tile
+// plus a one-cell ring (66 x 10 cells) from a three-plane LDS ring of the
+// loaded source (68 x 12 cells); its result u1 goes to a second three-plane
+// LDS ring and never to HBM.  The u1 values on the ring are stage 1 at the
+// periodic position -- the x/y boundary update of u1 -- and the planes 0 and
+// nz+1 of u1 and w1 on plane nz follow the boundary rules
+// (src/mod_updateBound.cc:92,123,137), as the stage-2 loads of k_rk rebuild
+// them.  Stage 2's point values are the loaded source at the tile point: the
+// same projection and forcing k_rk's stage 2 applies to them.  Per point, u,
+// v, w and p are read once and the new u, v, w written once.  The result goes
+// to separate arrays, since other blocks still read the source.  With `acc`
+// the block sums the new u times dzflzi(k) in FP64 into partial[block].
+template <int TY>
+__global__ __launch_bounds__(kRkTx* TY) void k_rk12(
+    Geom g, const float* __restrict__ su, const float* __restrict__ sv,
+    const float* __restrict__ sw, const float* __restrict__ pp, float* __restrict__ du,
+    float* __restrict__ dv, float* __restrict__ dw, const float* __restrict__ dzc_inv, float dt,
+    float dtdxi, float dtdyi, float bc_u, const double* __restrict__ force, int forced_mask,
+    const RkCoef* __restrict__ coef1, const RkCoef* __restrict__ coef2, RkUniform cu1,
+    RkUniform cu2, int crf, int kchunk, int ntx, int nty, int acc,
+    const double* __restrict__ dzflzi, double* __restrict__ partial) {
+  constexpr int TX = kRkTx, NT = TX * TY;
+  constexpr int RX = TX + 2, RY = TY + 2, RPL = RX * RY;  // u1 region: tile + 1
+  constexpr int UX = TX + 4, UY = TY + 4, UPL = UX * UY;  // source: tile + 2
+  constexpr int NPF = (UPL + NT - 1) / NT;
+  constexpr int NRING = RPL - NT;  // region cells outside the tile
+  static_assert(NRING == 2 * RX + 2 * TY && NRING <= NT, "one ring cell per thread");
+  __shared__ float ssrc[3][3][UPL];  // [field][slot][cell]; cell (cx, cy) = (i0-1+cx, j0-1+cy)
+  __shared__ float su1[3][3][RPL];   // [field][plane % 3][cell]; cell (rx, ry) = (i0+rx, j0+ry)
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
+  const int kc = (int)(b / (unsigned)(ntx * nty));
+  const int i0 = ti * TX, j0 = tj * TY;
+  const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
+  const int i = i0 + tx + 1, j = j0 + ty + 1;
+  const bool active = i <= g.nx && j <= g.ny;
+  const bool fu = forced_mask & 1, fv = forced_mask & 2, fw = forced_mask & 4;
+  const float f_u = fu ? (float)force[0] : 0.f, f_v = fv ? (float)force[1] : 0.f,
+              f_w = fw ? (float)force[2] : 0.f;
+  if (k0 > k1) {
+    if (acc && t == 0) partial[b] = 0.0;
+    return;
+  }
+  auto wrap = [](int x, int n) { return ((x - 1) % n + n) % n + 1; };
+  int gaddr[NPF];  // plane-relative address of the periodic image (-1: past the tile)
+#pragma unroll
+  for (int n = 0; n < NPF; ++n) {
+    const int e = t + n * NT;
+    gaddr[n] = e < UPL ? wrap(i0 - 1 + e % UX, g.nx) + (int)g.sx * wrap(j0 - 1 + e / UX, g.ny) : -1;
+  }
+  // source plane kk with the rules of k_rk's folded, projected stage-1 loads
+  float pre[3][NPF];
+  auto fetch = [&](int kk) {
+    const bool ghost = kk == 0 || kk == g.nz + 1;
+    const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
+    const bool wzero = kk == 0 || kk >= g.nz;
+    const float dzci = dzc_inv[ks];
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        const long a = gaddr[n] + g.sxy * ks;
+        float qu = su[a], qv = sv[a], qw = wzero ? 0.f : sw[a];
+        const float p0 = pp[a];
+        qu = qu - (pp[a + 1] - p0) * dtdxi;
+        qv = qv - (pp[a + g.sx] - p0) * dtdyi;
+        if (!wzero) qw = qw - (pp[a + g.sxy] - p0) * dt * dzci;
+        if (fu) qu = qu + f_u;
+        if (fv) qv = qv + f_v;
+        if (fw && !wzero) qw = qw + f_w;
+        if (ghost) {
+          qu = 2.f * bc_u - qu;  // src/mod_updateBound.cc:92,123
+          qv = 2.f * 0.f - qv;
+        }
+        pre[0][n] = qu;
+        pre[1][n] = qv;
+        pre[2][n] = qw;
+      }
+  };
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        const int e = t + n * NT;
+        ssrc[0][slot][e] = pre[0][n];
+        ssrc[1][slot][e] = pre[1][n];
+        ssrc[2][slot][e] = pre[2][n];
+      }
+  };
+  // this thread's stage-1 cells: A = its own tile point, B = one ring cell (t < NRING)
+  const int ra = (ty + 1) * RX + tx + 1;
+  int bx, by;
+  if (t < RX) {
+    bx = t;
+    by = 0;
+  } else if (t < 2 * RX) {
+    bx = t - RX;
+    by = RY - 1;
+  } else if (t < 2 * RX + TY) {
+    bx = 0;
+    by = t - 2 * RX + 1;
+  } else {
+    bx = RX - 1;
+    by = t - 2 * RX - TY + 1;
+  }
+  const bool has_b = t < NRING;
+  const int rb = by * RX + bx;
+  int sm = 0, s0 = 1, sp = 2;  // source slots of planes q-1, q, q+1
+  const int qs = max(k0 - 1, 1), qe = k1 + 1;  // u1 planes formed: stage 1 up to nz, rules at nz+1
+  fetch(qs - 1);
+  commit(0);
+  fetch(qs);
+  commit(1);
+  fetch(qs + 1);
+  commit(2);
+  __syncthreads();
+  ZConv za{}, zb{}, z2{};  // carried lower-face products: stage 1 at A and B, stage 2
+  // stage 1 of region cell (rx, ry) on plane q, w1 = 0 on the wall plane nz.
+  // The carry restarts where k_rk's stage 1 restarts it (the first plane of
+  // each chunk of kchunk planes), and plane k0-1 continues the chain of the
+  // chunk below, so u1 is bit-identical to k_rk's.
+  auto stage1 = [&](int q, int rx, int ry, ZConv& z, float& o1, float& o2, float& o3) {
+    const int c = (ry + 1) * UX + rx + 1;
+    const PlaneAcc<float, UX> U{ssrc[0][sm], ssrc[0][s0], ssrc[0][sp], c};
+    const PlaneAcc<float, UX> V{ssrc[1][sm], ssrc[1][s0], ssrc[1][sp], c};
+    const PlaneAcc<float, UX> W{ssrc[2][sm], ssrc[2][s0], ssrc[2][sp], c};
+    const RkCoef cq = coef1[q];
+    if ((q - 1) % kchunk == 0) {
+      z = zconv_lower(U, V, W, cq);
+    } else if (q == qs) {
+      const PlaneAcc<float, UX> Ub{ssrc[0][sm], ssrc[0][sm], ssrc[0][s0], c};
+      const PlaneAcc<float, UX> Vb{ssrc[1][sm], ssrc[1][sm], ssrc[1][s0], c};
+      const PlaneAcc<float, UX> Wb{ssrc[2][sm], ssrc[2][sm], ssrc[2][s0], c};
+      z = zconv_upper(Ub, Vb, Wb, coef1[q - 1]);
+    }
+    float iu, iv, iw;
+    if (crf)
+      tendency_fast<true, true>(U, V, W, cq, cu1, iu, iv, iw, &z);
+    else
+      tendency_fast<false, true>(U, V, W, cq, cu1, iu, iv, iw, &z);
+    o1 = U(0, 0, 0) + iu;
+    o2 = V(0, 0, 0) + iv;
+    o3 = q >= g.nz ? 0.f : W(0, 0, 0) + iw;
+  };
+  // u1 of plane q at region cell r; with q == 1 also plane 0 by the rules
+  auto put1 = [&](int q, int r, float o1, float o2, float o3) {
+    const int s = q % 3;
+    su1[0][s][r] = o1;
+    su1[1][s][r] = o2;
+    su1[2][s][r] = o3;
+    if (q == 1) {
+      su1[0][0][r] = 2.f * bc_u - o1;
+      su1[1][0][r] = 2.f * 0.f - o2;
+      su1[2][0][r] = 0.f;
+    }
+  };
+  // plane nz+1 of u1 by the rules from plane nz (the same thread wrote cell r of it)
+  auto ghost_top = [&](int r) {
+    const int s = g.nz % 3, d = (g.nz + 1) % 3;
+    su1[0][d][r] = 2.f * bc_u - su1[0][s][r];
+    su1[1][d][r] = 2.f * 0.f - su1[1][s][r];
+    su1[2][d][r] = 0.f;
+  };
+  const int cp = (ty + 2) * UX + tx + 2;  // the tile point in the source ring
+  double msum = 0.0;
+  for (int q = qs; q <= qe; ++q) {
+    // the next iteration's stage 1 reads source planes q .. q+2
+    const bool more = q + 1 <= qe && q + 1 <= g.nz;
+    const int k = q - 1;  // stage-2 plane
+    const bool s2 = k >= k0;
+    float pu = 0.f, pv = 0.f, pw = 0.f;
+    if (s2) {
+      pu = ssrc[0][sm][cp];
+      pv = ssrc[1][sm][cp];
+      pw = ssrc[2][sm][cp];
+    }
+    if (q <= g.nz) {
+      float o1, o2, o3;
+      stage1(q, tx + 1, ty + 1, za, o1, o2, o3);
+      put1(q, ra, o1, o2, o3);
+      if (has_b) {
+        stage1(q, bx, by, zb, o1, o2, o3);
+        put1(q, rb, o1, o2, o3);
+      }
+    } else {
+      ghost_top(ra);
+      if (has_b) ghost_top(rb);
+    }
+    __syncthreads();
+    // fetched here rather than before stage 1: its registers are then not
+    // live across both stages
+    if (more) fetch(q + 2);
+    if (s2 && active) {
+      const PlaneAcc<float, RX> U{su1[0][(k + 2) % 3], su1[0][k % 3], su1[0][(k + 1) % 3], ra};
+      const PlaneAcc<float, RX> V{su1[1][(k + 2) % 3], su1[1][k % 3], su1[1][(k + 1) % 3], ra};
+      const PlaneAcc<float, RX> W{su1[2][(k + 2) % 3], su1[2][k % 3], su1[2][(k + 1) % 3], ra};
+      const RkCoef cq = coef2[k];
+      if (k == k0) z2 = zconv_lower(U, V, W, cq);
+      float iu, iv, iw;
+      if (crf)
+        tendency_fast<true, true>(U, V, W, cq, cu2, iu, iv, iw, &z2);
+      else
+        tendency_fast<false, true>(U, V, W, cq, cu2, iu, iv, iw, &z2);
+      const long a = g.idx(i, j, k);
+      const float nu = (U(0, 0, 0) + pu) * 0.5f + iu;
+      du[a] = nu;
+      dv[a] = (V(0, 0, 0) + pv) * 0.5f + iv;
+      dw[a] = (W(0, 0, 0) + pw) * 0.5f + iw;
+      if (acc) msum += (double)nu * dzflzi[k];
+    }
+    if (more) commit(sm);  // source plane q+2 replaces q-1
+    __syncthreads();
+    const int tmp = sm;
+    sm = s0;
+    s0 = sp;
+    sp = tmp;
+  }
+  if (acc) {
+    // the block size is folded onto its largest power of two first
+    double* red = reinterpret_cast<double*>(&ssrc[0][0][0]);
+    static_assert(sizeof(ssrc) >= NT * sizeof(double), "reduction scratch");
+    constexpr int P2 = NT >= 1024 ? 1024 : (NT >= 512 ? 512 : 256);
+    static_assert(NT >= P2 && NT < 2 * P2, "block size");
+    red[t] = msum;
+    __syncthreads();
+    if (t < NT - P2) red[t] += red[t + P2];
+    __syncthreads();
+    for (int h = P2 / 2; h > 0; h >>= 1) {
+      if (t < h) red[t] += red[t + h];
+      __syncthreads();
+    }
+    if (t == 0) partial[b] = red[0];
   }
 }
 
@@ -1277,6 +1540,46 @@ int rk2_rhs_t(pll_context* c, int defer, int corr, int acc) {
   return PLL_OK;
 }
 
+// Both stages of an FP32-mixed step inside a run (the previous step's
+// projection, forcing and boundary update applied on load) into the spare
+// velocity buffers, then swapped in; acc: the bulk mean of the new u and the
+// force of u go to d_scalar[0], [3].
+constexpr int kRk12Ty = 8;  // tile rows of the fused stages
+int rk12_f32(pll_context* c, int acc) {
+  const TileGrid t = tile_grid(c, kRk12Ty);
+  if (acc && (long)t.blocks() > c->npartial)
+    return fail(PLL_ERR_STATE, "reduction scratch smaller than the stencil grid");
+  if (!c->fld_alt[0] || !c->fld_alt[1] || !c->fld_alt[2])
+    return fail(PLL_ERR_STATE, "no spare velocity buffers");
+  const MeshT<float> m = mesh_of<float>(c);
+  const RkCoef* coef = static_cast<const RkCoef*>(c->d_rkcoef);
+  const double* dzflzi = c->d_mesh64 + 3 * (c->g.nz + 2);
+  {
+    TimerScope ts(c, K_RK12);
+    hipLaunchKernelGGL(k_rk12<kRk12Ty>, dim3(t.blocks()), dim3(kRkTx * kRk12Ty), 0, c->stream, c->g,
+                       F<float>(c, PLL_U), F<float>(c, PLL_V), F<float>(c, PLL_W),
+                       F<float>(c, PLL_P), static_cast<float*>(c->fld_alt[0]),
+                       static_cast<float*>(c->fld_alt[1]), static_cast<float*>(c->fld_alt[2]),
+                       m.dzc_inv, (float)c->dt, (float)(c->dt * c->dx_inv),
+                       (float)(c->dt * c->dy_inv),  // src/mod_calcVel.cc:256-258
+                       (float)(0.0 - c->u_crf),     // src/mod_updateBound.cc:80
+                       c->d_scalar + 3, forced_mask(c), coef, coef + (c->g.nz + 2),
+                       rk_uniform(c, 1), rk_uniform(c, 2), (int)(c->u_crf != 0.0), t.kchunk,
+                       t.ntx, t.nty, acc, dzflzi, c->d_partial);
+  }
+  PLL_HIP_TRY(hipGetLastError());
+  for (int d = 0; d < 3; ++d) std::swap(c->fld[PLL_U + d], c->fld_alt[d]);
+  if (acc) {
+    // forceVel's bulk mean and force of u (src/mod_calcVel.cc:287,297)
+    TimerScope ts(c, K_MEAN_FINAL);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kRedThreads), 0, c->stream, c->d_partial,
+                       (long)t.blocks(), (double)c->g.nx, (double)c->g.ny, c->d_scalar + 0,
+                       c->vel_force[0], c->d_scalar + 3);
+    PLL_HIP_TRY(hipGetLastError());
+  }
+  return PLL_OK;
+}
+
 template <typename T>
 int rhs_t(pll_context* c) {
   const TileGrid t = tile_grid(c);
@@ -1498,6 +1801,10 @@ int launch_rk(pll_context* c, int stage, int fold, int corr, int acc, int bcload
                       rk_t<float>(c, stage, fold, corr, acc, bcload));
 }
 int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
+int launch_rk12(pll_context* c, int acc) {
+  if (c->prec != PLL_FP32_MIXED) return fail(PLL_ERR_STATE, "the fused stages are the FP32-mixed step's");
+  return rk12_f32(c, acc);
+}
 int launch_rk2_rhs(pll_context* c, int defer, int corr, int acc) {
   if (c->prec != PLL_FP64) return fail(PLL_ERR_STATE, "the fused stage 2 + calcRHS is the FP64 step's");
   return rk2_rhs_t<double>(c, defer, corr, acc);

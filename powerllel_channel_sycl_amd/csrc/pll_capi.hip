@@ -15,7 +15,7 @@ const char* const kKernelNames[K_COUNT] = {
     "rk1",         "rk2",       "calc_rhs",   "fft_x_fwd",    "fft_y_fwd", "solve_trid",
     "fft_y_bwd",   "fft_x_bwd", "correct_vel", "mean_partial", "mean_final", "force_add",
     "bc_vel",      "bc_p",      "check",      "init",         "misc",        "rk2_rhs",
-    "fft_x_fwd_rhs", "rk_closure"};
+    "fft_x_fwd_rhs", "rk_closure", "rk12"};
 
 namespace {
 thread_local std::string g_last_error;
@@ -324,10 +324,17 @@ int step_impl(pll_context* c, bool first, bool last) {
     // calcRHS runs inside the forward x transform's loads (:133-137).
     // The boundary updates of u1 (:121) and u (:129) are not run either:
     // stage 2 and calcRHS rebuild the halos they read from the boundary rules.
+    // Inside a run both stages run in one kernel (u1 stays in LDS); the
+    // first step (nothing deferred) and the last (u1 is left in its arrays)
+    // run them separately.
     const int corr = fold;
     const int acc = (!last && c->is_forced[0]) ? 1 : 0;
-    if ((rc = launch_rk(c, 1, fold, corr, 0))) return rc;       // :117
-    if ((rc = launch_rk(c, 2, fold, corr, acc, 1))) return rc;  // :125
+    if (fold && !last) {
+      if ((rc = launch_rk12(c, acc))) return rc;  // :117-125
+    } else {
+      if ((rc = launch_rk(c, 1, fold, corr, 0))) return rc;       // :117
+      if ((rc = launch_rk(c, 2, fold, corr, acc, 1))) return rc;  // :125
+    }
     c->rhs_bcfold = 1;
     rc = launch_poisson_step(c, true);  // :133-137
     c->rhs_bcfold = 0;

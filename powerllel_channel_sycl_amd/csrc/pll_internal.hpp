@@ -75,6 +75,7 @@ enum KernelId {
   K_RK2_RHS,        // stage 2 fused with calcRHS
   K_FFT_X_FWD_RHS,  // calcRHS fused into the forward x transform
   K_RK_CLOSURE,     // RK stage with the learned eddy viscosity
+  K_RK12,           // both RK stages of the FP32-mixed step
   K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
@@ -215,6 +216,7 @@ int launch_calc_rhs(pll_context* c);
 // the previous step's force, corr applies its projection on load, acc reduces
 // the bulk mean of the new u and the force of u into d_scalar[0], [3].
 int launch_rk2_rhs(pll_context* c, int defer, int corr = 0, int acc = 0);
+int launch_rk12(pll_context* c, int acc);
 // with_mean: also reduce the bulk mean of the corrected u and store the force
 // of u (vel_force[0] - mean) on the device.
 int launch_correct_vel(pll_context* c, bool with_mean);

@@ -53,4 +53,6 @@ def test_step_bytes_add_up():
               "correct_vel", "mean_partial", "force_add"):
         assert 2 * f[n] == k[n], n
     assert f["rk2"] == k["rk2"] // 2 + 4
+    # the fused FP32-mixed stages read u, v, w, p and write u, v, w
+    assert f["rk12"] == 4 * 4 + 3 * 4
     assert b.STAT_BYTES == {"fp64": 208, "fp32_mixed": 192}
