@@ -598,6 +598,78 @@ __global__ __launch_bounds__(256) void k_samples(
   y[n] = (float)fmin(nu / nus, ymax);
 }
 
+// ---- the deferred projection, forcing and boundary update -----------------
+// In the closure's step sequence the previous step's velocity has neither its
+// projection nor its forcing nor its boundary update: its values are rebuilt
+// from the stored ones as k_rk's folded loads do.  With `bcload`: x/y halo
+// cells read their periodic image, the ghost planes 0 and nz+1 follow the
+// boundary rules from planes 1 and nz (2 bc - u, -v;
+// src/mod_updateBound.cc:92,123) and w is 0 on planes 0, nz and nz+1 (:137);
+// with `corr` the value first becomes u - dt grad(p) (src/mod_calcVel.cc:272-274,
+// k_correct's association), then the deferred forcing of each component in
+// fmask (device scalars force[0..2]) is added.  Without `bcload` the stored
+// value is read.  p's x halos must hold their periodic values (the step's
+// inverse x transform writes them); its y halo is read at its image and its z
+// ghost planes are not needed (w is 0 on the planes that would read them), so
+// updateBoundP need not have run.
+template <typename T>
+struct VelRules {
+  int bcload, corr;
+  const T* p;              // previous pressure, valid halos (corr)
+  const T* dzc_inv;        // (corr)
+  T dtdxi, dtdyi, dt;      // (corr)
+  const double* force;     // deferred forcing of u, v, w (fmask)
+  int fmask;
+  T bc_u;                  // u's wall value, src/mod_updateBound.cc:80
+};
+
+// component fc (0 u, 1 v, 2 w) of field q at (i, j, k), 0 <= i <= nx+1 etc.
+template <typename T>
+__device__ __forceinline__ T rule_load(const T* __restrict__ q, int fc, int i, int j, int k,
+                                       const Geom& g, const VelRules<T>& r) {
+  if (!r.bcload) return q[g.idx(i, j, k)];
+  const int ii = i == 0 ? g.nx : (i == g.nx + 1 ? 1 : i);
+  const int jj = j == 0 ? g.ny : (j == g.ny + 1 ? 1 : j);
+  const bool ghost = k == 0 || k == g.nz + 1;
+  const int ks = ghost ? (k == 0 ? 1 : g.nz) : k;
+  if (fc == 2 && (k == 0 || k >= g.nz)) return T(0);
+  const long a = g.idx(ii, jj, ks);
+  T v = q[a];
+  if (r.corr) {
+    const T p0 = r.p[a];
+    if (fc == 0)
+      v = v - (r.p[a + 1] - p0) * r.dtdxi;
+    else if (fc == 1)
+      v = v - (r.p[jj == g.ny ? a - (long)(g.ny - 1) * g.sx : a + g.sx] - p0) * r.dtdyi;
+    else
+      v = v - (r.p[a + g.sxy] - p0) * r.dt * r.dzc_inv[ks];
+  }
+  if ((r.fmask >> fc) & 1) v = v + (T)r.force[fc];
+  if (ghost) v = fc == 0 ? T(2) * r.bc_u - v : T(2) * T(0) - v;
+  return v;
+}
+
+// The previous step's projection, forcing and boundary update of u, v, w
+// (src/mod_calcVel.cc:253-320, src/mod_updateBound.cc:229-251) in one pass
+// over every cell, halos included, into separate arrays (the source's halo
+// images must stay unprojected while other threads read them).
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_deferred(Geom g, const T* __restrict__ u,
+                                                        const T* __restrict__ v,
+                                                        const T* __restrict__ w,
+                                                        T* __restrict__ du, T* __restrict__ dv,
+                                                        T* __restrict__ dw, VelRules<T> r) {
+  const long n = (long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % (g.nx + 2)), j = (int)((e / (g.nx + 2)) % (g.ny + 2));
+    const int k = (int)(e / ((long)(g.nx + 2) * (g.ny + 2)));
+    const long a = g.idx(i, j, k);
+    du[a] = rule_load<T>(u, 0, i, j, k, g, r);
+    dv[a] = rule_load<T>(v, 1, i, j, k, g, r);
+    dw[a] = rule_load<T>(w, 2, i, j, k, g, r);
+  }
+}
+
 // ---- the closure stage: RK stage (src/mod_calcVel.cc:27-242 in the
 // reference's arithmetic) plus the divergence of the eddy-viscosity stress
 // 2 nu_t S_ij, with nu_t = nus * softplus(MLP(features)).  k_nut evaluates the
@@ -912,12 +984,52 @@ __global__ __launch_bounds__(64 * kWaves) void k_nut(Geom g, const T* __restrict
   }
 }
 
+// u1's boundary update (src/mod_updateBound.cc:229-251, k_bc_vel's closed
+// form) written by the interior cells it copies: every x/y halo image of the
+// cell on its plane, the ghost planes 0 and nz+1 by the wall rules, and w = 0
+// on the wall plane nz.  Each halo cell has one interior source, so exactly one
+// thread writes it.
+template <typename T>
+__device__ __forceinline__ void store_with_halos(const Geom& g, T* du, T* dv, T* dw, int i,
+                                                 int j, int k, T a, T b, T c, T bc_u) {
+  const T zero = T(0), two = T(2);
+  if (k == g.nz) c = zero;
+  const int xi[3] = {i, i == 1 ? g.nx + 1 : -1, i == g.nx ? 0 : -1};
+  const int yj[3] = {j, j == 1 ? g.ny + 1 : -1, j == g.ny ? 0 : -1};
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (xi[p] < 0 || yj[q] < 0) continue;
+      const long d = g.idx(xi[p], yj[q], k);
+      du[d] = a;
+      dv[d] = b;
+      dw[d] = c;
+      if (k == 1) {
+        const long e = g.idx(xi[p], yj[q], 0);
+        du[e] = two * bc_u - a;
+        dv[e] = two * zero - b;
+        dw[e] = zero;
+      }
+      if (k == g.nz) {
+        const long e = g.idx(xi[p], yj[q], g.nz + 1);
+        du[e] = two * bc_u - a;
+        dv[e] = two * zero - b;
+        dw[e] = zero;
+      }
+    }
+}
+
+// With `halo` stage 1 also writes u1's boundary (store_with_halos).  With
+// `acc` stage 2 sums its new u times dzflzi(k) in FP64 into partial[block]
+// (the bulk mean of the next projected velocity, as k_rk's).
 template <typename T, int STAGE>
 __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
     Geom g, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
     T* __restrict__ du, T* __restrict__ dv, T* __restrict__ dw, MeshT<T> m, T dxi, T dyi,
-    T rei, T dt, T ucrf, const float* __restrict__ nut, int ntx, int nty) {
-  __shared__ float nu[kExt];
+    T rei, T dt, T ucrf, const float* __restrict__ nut, int ntx, int nty, int halo, T bc_u,
+    int acc, const double* __restrict__ dzflzi, double* __restrict__ partial) {
+  __shared__ __align__(16) float nu[kExt];  // 16-byte aligned: reused as FP64 sums
   const int b = blockIdx.x;
   const int ti = b % ntx, tj = (b / ntx) % nty, tk = b / (ntx * nty);
   const int i0 = ti * kLx, j0 = tj * kLy, k0 = tk * kLz;  // extended cell (0,0,0) = (i0,j0,k0)
@@ -938,16 +1050,18 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
   // current plane is computed.  Planes beyond nz (a ragged last block) are
   // computed on plane nz's values and not stored.
   const int tx = threadIdx.x % kLx, ty = threadIdx.x / kLx;
-  const int i = i0 + tx + 1, j = j0 + ty + 1;
-  if (i > g.nx || j > g.ny) return;
+  const bool inside = i0 + tx + 1 <= g.nx && j0 + ty + 1 <= g.ny;
+  // threads past the grid work on its last cell and store nothing
+  const int i = inside ? i0 + tx + 1 : g.nx, j = inside ? j0 + ty + 1 : g.ny;
   const T two = T(2), qtr = T(0.25);
+  double msum = 0.0;
   auto plane_of = [&](int kz) { const int k = k0 + kz + 1; return k < g.nz ? k : g.nz; };
   LesPlane<T> cur, nxt;
   load_les_plane<T, STAGE == 2>(cur, su, sv, sw, du, dv, dw, g.idx(i, j, plane_of(0)), g.sx, g.sxy);
 #pragma unroll
   for (int kz = 0; kz < kLz; ++kz) {
     const int k = plane_of(kz);
-    const bool live = k0 + kz + 1 <= g.nz;
+    const bool live = inside && k0 + kz + 1 <= g.nz;
     if (kz + 1 < kLz)
       load_les_plane<T, STAGE == 2>(nxt, su, sv, sw, du, dv, dw, g.idx(i, j, plane_of(kz + 1)),
                                     g.sx, g.sxy);
@@ -1012,16 +1126,38 @@ __global__ __launch_bounds__(64 * kWaves) void k_rk_les(
     }
     if (live) {
       if (STAGE == 1) {
-        du[a] = U(0, 0, 0) + dt * tu + dt * s_u;
-        dv[a] = V(0, 0, 0) + dt * tv + dt * s_v;
-        dw[a] = W(0, 0, 0) + dt * tw + dt * s_w;
+        const T n1 = U(0, 0, 0) + dt * tu + dt * s_u;
+        const T n2 = V(0, 0, 0) + dt * tv + dt * s_v;
+        const T n3 = W(0, 0, 0) + dt * tw + dt * s_w;
+        if (halo) {
+          store_with_halos(g, du, dv, dw, i, j, k, n1, n2, n3, bc_u);
+        } else {
+          du[a] = n1;
+          dv[a] = n2;
+          dw[a] = n3;
+        }
       } else {
-        du[a] = (U(0, 0, 0) + dt * tu + cur.o[0] + dt * s_u) * T(0.5);
+        const T nu = (U(0, 0, 0) + dt * tu + cur.o[0] + dt * s_u) * T(0.5);
+        du[a] = nu;
         dv[a] = (V(0, 0, 0) + dt * tv + cur.o[1] + dt * s_v) * T(0.5);
         dw[a] = (W(0, 0, 0) + dt * tw + cur.o[2] + dt * s_w) * T(0.5);
+        if (acc) msum += (double)nu * dzflzi[k];
       }
     }
     if (kz + 1 < kLz) cur = nxt;
+  }
+  if (STAGE == 2 && acc) {
+    // nu (the eddy viscosity block) is free: every thread passed its reads
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(nu);
+    static_assert(sizeof(nu) >= 64 * kWaves * sizeof(double), "reduction scratch");
+    red[threadIdx.x] = msum;
+    __syncthreads();
+    for (int h = 32 * kWaves; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[b] = red[0];
   }
 }
 
@@ -1043,6 +1179,7 @@ struct pll_closure {
   hipStream_t stream = nullptr;
   int device = 0;              // the device current at creation (that of stream)
   hipEvent_t ready = nullptr;  // orders a context's closure stages after training
+  long version = 0;            // bumped by every asynchronous change of the weights
   // feature and output scales: inputs g*gs and (wall distance)*ys, nu_t = nus*softplus(o)
   double gs = 1.0, ys = 1.0, nus = 1.0;
 };
@@ -1178,11 +1315,21 @@ T* fld(const pll_context* c, int f) {
   return static_cast<T*>(c->fld[f]);
 }
 
+// The closure stage.  Stage 1 with fold (and corr) first applies the previous
+// step's forcing (and projection) and boundary update in one pass
+// (k_apply_deferred), and with bcload writes u1's boundary update itself;
+// stage 2 with acc sums the bulk-mean partials of its new u for the next
+// step's forcing.  The sources are read as stored, halos included.
 template <typename T>
-int rk_closure_t(pll_context* c, int stage) {
+int rk_closure_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
   pll_closure* cl = c->closure;
-  PLL_HIP_TRY(hipEventRecord(cl->ready, cl->stream));
-  PLL_HIP_TRY(hipStreamWaitEvent(c->stream, cl->ready, 0));
+  if (c->closure_version != cl->version) {
+    // the weights changed on the closure's stream (training) since this
+    // context last waited for them
+    PLL_HIP_TRY(hipEventRecord(cl->ready, cl->stream));
+    PLL_HIP_TRY(hipStreamWaitEvent(c->stream, cl->ready, 0));
+    c->closure_version = cl->version;
+  }
   const Geom g = c->g;
   const int ntx = (g.nx + kLx - 1) / kLx, nty = (g.ny + kLy - 1) / kLy, ntz = (g.nz + kLz - 1) / kLz;
   const dim3 grid((unsigned)ntx * nty * ntz), block(64 * kWaves);
@@ -1196,21 +1343,58 @@ int rk_closure_t(pll_context* c, int stage) {
   // tile, up to kMaxWaves waves), then the stage with that eddy viscosity
   const long ncell = c->nspec;
   const long nblk = std::min((ncell + kNutChunk - 1) / kNutChunk, 512L);
-  TimerScope ts(c, K_RK_CLOSURE);
-  hipLaunchKernelGGL((k_nut<T>), dim3((unsigned)nblk), block, 0, c->stream, g,
-                     su, sv, sw, mesh_t<T>(c), feat_mesh<T>(c), (float)c->dx_inv, (float)c->dy_inv,
-                     (float)(0.25 * c->dx_inv), (float)(0.25 * c->dy_inv), weights(cl), (float)cl->gs,
-                     (float)cl->ys, nus, c->d_nut, ncell);
-  if (stage == 1)
-    hipLaunchKernelGGL((k_rk_les<T, 1>), grid, block, 0, c->stream, g, su, sv, sw,
-                       fld<T>(c, PLL_U1), fld<T>(c, PLL_V1), fld<T>(c, PLL_W1), mesh_t<T>(c), dxi,
-                       dyi, rei, dt, ucrf, c->d_nut, ntx, nty);
-  else
-    hipLaunchKernelGGL((k_rk_les<T, 2>), grid, block, 0, c->stream, g, su, sv, sw,
-                       fld<T>(c, PLL_U), fld<T>(c, PLL_V), fld<T>(c, PLL_W), mesh_t<T>(c), dxi, dyi,
-                       rei, dt, ucrf, c->d_nut, ntx, nty);
+  if (acc && (stage != 2 || (long)grid.x > c->npartial))
+    return fail(PLL_ERR_STATE, "the bulk-mean partials are stage 2's and must fit the scratch");
+  if (corr && !fold) return fail(PLL_ERR_STATE, "deferred projection needs folded loads");
+  if (bcload && stage != 1) return fail(PLL_ERR_STATE, "the closure's stage 2 reads an updated u1");
+  const T bc_u = (T)(0.0 - c->u_crf);  // src/mod_updateBound.cc:80
+  if (stage == 1 && fold) {
+    // the previous step's projection, forcing and boundary update into the
+    // spare velocity buffers, which then become u, v, w
+    if (!c->fld_alt[0] || !c->fld_alt[1] || !c->fld_alt[2])
+      return fail(PLL_ERR_STATE, "no spare velocity buffers");
+    const int fmask = (c->is_forced[0] ? 1 : 0) | (c->is_forced[1] ? 2 : 0) | (c->is_forced[2] ? 4 : 0);
+    const VelRules<T> r{1, corr, fld<T>(c, PLL_P), mesh_t<T>(c).dzc_inv,
+                        (T)(c->dt * c->dx_inv), (T)(c->dt * c->dy_inv),  // src/mod_calcVel.cc:256-258
+                        dt, c->d_scalar + 3, fmask, bc_u};
+    const long n = (long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    TimerScope ts(c, K_CORRECT);
+    hipLaunchKernelGGL((k_apply_deferred<T>), dim3((unsigned)std::min((n + 255) / 256, 8192L)),
+                       dim3(256), 0, c->stream, g, su, sv, sw, static_cast<T*>(c->fld_alt[0]),
+                       static_cast<T*>(c->fld_alt[1]), static_cast<T*>(c->fld_alt[2]), r);
+    PLL_HIP_TRY(hipGetLastError());
+    for (int d = 0; d < 3; ++d) std::swap(c->fld[PLL_U + d], c->fld_alt[d]);
+    su = fld<T>(c, PLL_U);
+    sv = fld<T>(c, PLL_V);
+    sw = fld<T>(c, PLL_W);
+  }
+  const double* dzflzi = c->d_mesh64 + 3 * (c->g.nz + 2);
+  {
+    TimerScope ts(c, K_RK_CLOSURE);
+    hipLaunchKernelGGL((k_nut<T>), dim3((unsigned)nblk), block, 0, c->stream, g, su, sv, sw,
+                       mesh_t<T>(c), feat_mesh<T>(c), (float)c->dx_inv, (float)c->dy_inv,
+                       (float)(0.25 * c->dx_inv), (float)(0.25 * c->dy_inv), weights(cl),
+                       (float)cl->gs, (float)cl->ys, nus, c->d_nut, ncell);
+    if (stage == 1)
+      hipLaunchKernelGGL((k_rk_les<T, 1>), grid, block, 0, c->stream, g, su, sv, sw,
+                         fld<T>(c, PLL_U1), fld<T>(c, PLL_V1), fld<T>(c, PLL_W1), mesh_t<T>(c), dxi,
+                         dyi, rei, dt, ucrf, c->d_nut, ntx, nty, bcload, bc_u, 0, dzflzi,
+                         c->d_partial);
+    else
+      hipLaunchKernelGGL((k_rk_les<T, 2>), grid, block, 0, c->stream, g, su, sv, sw,
+                         fld<T>(c, PLL_U), fld<T>(c, PLL_V), fld<T>(c, PLL_W), mesh_t<T>(c), dxi,
+                         dyi, rei, dt, ucrf, c->d_nut, ntx, nty, 0, bc_u, acc, dzflzi,
+                         c->d_partial);
+  }
   PLL_HIP_TRY(hipGetLastError());
-  return PLL_OK;
+  // forceVel's bulk mean and force of u (src/mod_calcVel.cc:287,297)
+  return acc ? launch_force_from_partials(c, (long)grid.x) : PLL_OK;
+}
+
+// Blocks of the closure stage kernel (its bulk-mean partials must fit d_partial).
+long closure_stage_blocks(const pll_context* c) {
+  const Geom& g = c->g;
+  return (long)((g.nx + kLx - 1) / kLx) * ((g.ny + kLy - 1) / kLy) * ((g.nz + kLz - 1) / kLz);
 }
 
 // The fine and coarse grids nest: integer factors, the same box, and every
@@ -1279,11 +1463,15 @@ int filter(pll_context* f, pll_context* c) {
 }  // namespace
 
 namespace pll {
-int launch_rk_closure(pll_context* c, int stage) {
+int launch_rk_closure(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
   int rc = feat_setup_any(c);
   if (rc) return rc;
   if (!c->d_nut) PLL_HIP_TRY(ctx_malloc(c, &c->d_nut, (size_t)c->nspec * sizeof(float)));
-  return c->prec == PLL_FP64 ? rk_closure_t<double>(c, stage) : rk_closure_t<float>(c, stage);
+  return c->prec == PLL_FP64 ? rk_closure_t<double>(c, stage, fold, corr, acc, bcload)
+                             : rk_closure_t<float>(c, stage, fold, corr, acc, bcload);
+}
+bool closure_fused_available(const pll_context* c) {
+  return closure_stage_blocks(c) <= c->npartial;
 }
 }  // namespace pll
 
@@ -1363,6 +1551,7 @@ int pll_closure_attach(pll_context* ctx, pll_closure* c) {
     PLL_HIP_TRY(hipMemsetAsync(ctx->d_nut, 0, (size_t)ctx->nspec * sizeof(float), ctx->stream));
   }
   ctx->closure = c;
+  ctx->closure_version = -1;
   return PLL_OK;
 }
 
@@ -1541,6 +1730,7 @@ int pll_closure_train_step(pll_closure* c, const float* x, const float* y, long 
                      c->m1, c->m2, c->chunks, nchunk, c->grad, (float)lr, (float)b1, (float)b2,
                      1e-8f, c1, c2, c->wb);
   PLL_HIP_TRY(hipGetLastError());
+  c->version += 1;
   return PLL_OK;
 }
 

@@ -1792,15 +1792,21 @@ int nan_t(pll_context* c, int field, int* passed) {
 int rk_setup(pll_context* c) { return c->prec == PLL_FP64 ? PLL_OK : rk_coef_setup(c); }
 
 int launch_rk(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
-  if (c->closure) {
-    if (fold || corr || acc || bcload)
-      return fail(PLL_ERR_STATE, "the closure stage takes the plain stage sequence only");
-    return launch_rk_closure(c, stage);
-  }
+  if (c->closure) return launch_rk_closure(c, stage, fold, corr, acc, bcload);
   return PLL_DISPATCH(c, rk_t<double>(c, stage, fold, corr, acc, bcload),
                       rk_t<float>(c, stage, fold, corr, acc, bcload));
 }
 int launch_calc_rhs(pll_context* c) { return PLL_DISPATCH(c, rhs_t<double>(c), rhs_t<float>(c)); }
+int launch_force_from_partials(pll_context* c, long n) {
+  if (n < 1 || n > c->npartial || !c->d_partial)
+    return fail(PLL_ERR_STATE, "bulk-mean partials do not fit the scratch");
+  TimerScope ts(c, K_MEAN_FINAL);
+  hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(kRedThreads), 0, c->stream, c->d_partial, n,
+                     (double)c->g.nx, (double)c->g.ny, c->d_scalar + 0, c->vel_force[0],
+                     c->d_scalar + 3);
+  PLL_HIP_TRY(hipGetLastError());
+  return PLL_OK;
+}
 int launch_rk12(pll_context* c, int acc) {
   if (c->prec != PLL_FP32_MIXED) return fail(PLL_ERR_STATE, "the fused stages are the FP32-mixed step's");
   return rk12_f32(c, acc);

@@ -304,6 +304,27 @@ int finish_deferred(pll_context* c, bool last);
 
 int step_impl(pll_context* c, bool first, bool last) {
   int rc;
+  const bool fused = !c->is_forced[1] && !c->is_forced[2];
+  const int fold = (fused && !first) ? 1 : 0;
+  if (c->closure && fused && fft_x_rhs_available(c) && closure_fused_available(c)) {
+    // the closure stages (eddy viscosity evaluated on every cell of the
+    // stage's source) in a deferred sequence: the previous step's projection,
+    // forcing and boundary update (:145-153) in one pass before stage 1, u1's
+    // boundary update written by stage 1, the predictor's bulk mean for the
+    // forcing summed by stage 2, and calcRHS inside the forward x transform
+    // (which rebuilds u's boundary on load)
+    const int corr = fold;
+    const int acc = (!last && c->is_forced[0]) ? 1 : 0;
+    if ((rc = launch_rk(c, 1, fold, corr, 0, 1))) return rc;  // :145-153, :117, :121
+    if ((rc = launch_rk(c, 2, 0, 0, acc, 0))) return rc;      // :125
+    c->rhs_bcfold = 1;
+    rc = launch_poisson_step(c, true);  // :133-137
+    c->rhs_bcfold = 0;
+    if (rc) return rc;
+    // inside a run p's boundary update (:141) is left to the next step's
+    // deferred pass, which reads p's halos through their rules
+    return last ? finish_deferred(c, last) : PLL_OK;
+  }
   if (c->closure) {
     // the closure stages (eddy viscosity evaluated in the stage kernel), then
     // the reference sequence src/main.cc:121-153 unfused
@@ -315,8 +336,6 @@ int step_impl(pll_context* c, bool first, bool last) {
     if ((rc = launch_poisson_step(c, false))) return rc;  // :137
     return finish_step(c, false, last);
   }
-  const bool fused = !c->is_forced[1] && !c->is_forced[2];
-  const int fold = (fused && !first) ? 1 : 0;
   if (fused && c->prec != PLL_FP64 && fft_x_rhs_available(c)) {
     // FP32-mixed: the projection of step n (:145), its forcing (:149) and
     // boundary update (:153) are applied by step n+1's stage loads (corr);

@@ -145,6 +145,7 @@ struct pll_context {
   // learned closure (not owned): when attached, the RK stages add the
   // eddy-viscosity stress with the network evaluated in the stage kernel
   pll_closure* closure = nullptr;
+  long closure_version = -1;  // the closure's weight version this stream has waited for
   void* d_feat = nullptr;   // feature coefficients per plane (field precision), 3*(nz+2)
   float* d_nut = nullptr;   // eddy viscosity of the last closure stage, nx*ny*nz
 
@@ -207,8 +208,16 @@ struct TimerScope {
 // bcload (stage 2, FP32): rebuild the source's halos on load (u1's boundary
 // update not run)
 int launch_rk(pll_context* c, int stage, int fold, int corr = 0, int acc = 0, int bcload = 0);
-// the stage of a context with an attached closure (k_closure.hip)
-int launch_rk_closure(pll_context* c, int stage);
+// the stage of a context with an attached closure (k_closure.hip); fold,
+// corr, acc, bcload as for launch_rk
+int launch_rk_closure(pll_context* c, int stage, int fold = 0, int corr = 0, int acc = 0,
+                      int bcload = 0);
+// the closure stages' bulk-mean partials fit d_partial (the fused step sequence)
+bool closure_fused_available(const pll_context* c);
+// forceVel's bulk mean of u and its force from n FP64 block partials in
+// d_partial, into d_scalar[0] and d_scalar[3] (k_stencil.hip)
+int launch_force_from_partials(pll_context* c, long n);
+
 int rk_setup(pll_context* c);  // per-plane coefficients of the FP32-mixed stencil
 int launch_calc_rhs(pll_context* c);
 // RK stage 2 fused with calcRHS (FP64 step sequence only): writes the new u, v,

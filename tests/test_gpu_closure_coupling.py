@@ -176,6 +176,73 @@ def test_closure_steps_stay_divergence_free(precision):
     cl.close()
 
 
+def _closure_run(name, precision, seed, run):
+    """Fields after run(ch) from one seeded state with a closure attached."""
+    prm = params(name)
+    cl, _ = _closure_with_params(seed, prm["re"])
+    o = make_oracle(name)
+    u, v, w, p = (o.field() for _ in range(4))
+    o.init_flow(u, v, w, "poi", noise_intensity=0.1)
+    o.update_bound_vel(u, v, w)
+    o.update_bound_p(p)
+    with make_channel(name, precision=precision) as ch:
+        for f, a in zip("uvwp", (u, v, w, p)):
+            ch.upload(f, a)
+        ch.attach_closure(cl)
+        run(ch)
+        out = [ch.download(f) for f in "uvwp"]
+        ch.attach_closure(None)
+    cl.close()
+    return out
+
+
+@pytest.mark.parametrize("name,precision", [("r235", "fp64"), ("r3", "fp64"),
+                                            ("r235", "fp32_mixed")])
+def test_closure_multistep_folding_matches_single_steps(name, precision):
+    """With a closure attached, pll_step(n) applies each step's projection,
+    forcing and boundary update in one pass before the next step's stage 1,
+    sums the forcing's bulk mean from the predictor in stage 2 and runs calcRHS
+    inside the x transform (rebuilding u's boundary on load).
+    The projected values are the separate projection's; the bulk mean is the
+    same sum of other roundings, so n steps in one call match n calls of one
+    step to round-off."""
+    print(f"CASE closure folding {name} {precision}", flush=True)
+
+    def fused(ch):
+        ch.step(6)
+
+    def single(ch):
+        for _ in range(6):
+            ch.step(1)
+    a = _closure_run(name, precision, 14, fused)
+    b = _closure_run(name, precision, 14, single)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    for x, y, f in zip(a, b, "uvwp"):
+        np.testing.assert_allclose(x, y, rtol=0, atol=tol, err_msg=f)
+
+
+@pytest.mark.parametrize("name", ["r235", "gen"])
+def test_closure_step_matches_entry_point_sequence(name):
+    """pll_step with a closure attached (fused sequence where the x transform
+    has its calcRHS kernel, r235; the reference's sequence otherwise, gen)
+    matches the reference's call sequence through the entry points
+    (src/main.cc:116-154) to FP64 round-off."""
+    print(f"CASE closure entry sequence {name}", flush=True)
+
+    def step(ch):
+        ch.step(1)
+
+    def calls(ch):
+        ch.timeIntVelRK1(); ch.updateBoundVel(1); ch.timeIntVelRK2(); ch.updateBoundVel(0)
+        ch.calcRHS(); ch.execute_poisson_solver(); ch.updateBoundP(); ch.correctVel()
+        ch.forceVel(); ch.updateBoundVel(0)
+    a = _closure_run(name, "fp64", 15, step)
+    b = _closure_run(name, "fp64", 15, calls)
+    for x, y, f in zip(a, b, "uvwp"):
+        scale = max(np.max(np.abs(y)), 1e-30)
+        assert np.max(np.abs(x - y)) <= 1e-13 * scale, (f, np.max(np.abs(x - y)) / scale)
+
+
 def test_closure_detached_is_plain_step():
     """A detached closure leaves the plain FP64 step bit for bit."""
     name = "r235"
