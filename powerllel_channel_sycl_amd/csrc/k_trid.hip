@@ -133,7 +133,12 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
 // The same solve inside the step, partitioned so that the work array makes one
 // HBM pass each way.  A block owns kCw adjacent columns of one row j; each
 // column is cut into S segments of M planes, one thread per segment, and the
-// segment's right-hand side and reciprocal pivots stay in registers:
+// segment's right-hand side and reciprocal pivots stay in registers.  The
+// pivots are not read from a table: each segment continues the elimination's
+// pivot recurrence (init_poisson_solver, src/mod_poisson_solver.cc:273-282)
+// from the pivot of the plane below it, which the table `pstart` holds, with
+// the reference's operations (FP64 division), so every pivot is the one the
+// full table holds, bit for bit; the reciprocal is 0 for a zero pivot:
 //   A. forward elimination of the segment from a zero incoming value, with the
 //      running product g of the negated multipliers (the segment end's
 //      dependence on the incoming value);
@@ -146,6 +151,12 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
 // sequential sweep, and the pivots are applied as reciprocals, so the result
 // differs from k_trid's by FP64 round-off (it is not bit-identical).
 constexpr int kSegPlanes = 384;  // the deepest column the segments hold
+// planes per segment (at most kSegM) and segments per column (at most kSegS)
+constexpr int kSegM = 24, kSegS = 16;
+static_assert(kSegM * kSegS >= kSegPlanes, "the segments hold the deepest column");
+constexpr int seg_planes(int nz) {
+  return nz <= kSegS * (kSegM / 3) ? kSegM / 3 : (nz <= kSegS * (2 * kSegM / 3) ? 2 * kSegM / 3 : kSegM);
+}
 
 // The sequential combination of the S segment-end values, Y_q = hat_q + dep_q *
 // Y_{q-1} (forward, REV false) or from the top (REV true), by the threads of
@@ -177,42 +188,66 @@ __device__ __forceinline__ void combine(const double (*hat)[CW], const double (*
   }
 }
 
+// SegCoef: the diagonal before the eigenvalue b (nz), the compact eigenvalues
+// of x (cxn) and y (cyn), and the pivots of the planes below the segments,
+// pstart[s * cxn * cyn + cy * cxn + cx] = pivot of plane s*M - 1 (s >= 1).
+struct SegCoef {
+  const double* b;
+  const double* elx;
+  const double* ely;
+  const double* pstart;
+  int pin;  // the (0,0) column's last pivot is pinned to 0
+};
+
 template <typename T, int M, int CW, int S>
 __global__ __launch_bounds__(CW* S) void k_trid_seg(int nx, int ny, int nz, int nseg,
                                                         T* __restrict__ var,
                                                         const double* __restrict__ a,
                                                         const double* __restrict__ c,
-                                                        const double* __restrict__ rpiv, int cxn,
-                                                        int cyn) {
+                                                        SegCoef sg, int cxn, int cyn) {
   __shared__ double hat[S][CW];  // segment-end value from a zero incoming value
   __shared__ double dep[S][CW];  // its dependence on the incoming value
   __shared__ double inc[S][CW];  // the true value at the segment end
-  __shared__ double sa[kSegPlanes], sc[kSegPlanes];
+  __shared__ double sa[kSegPlanes], sb[kSegPlanes], sc[kSegPlanes];
   const int t = threadIdx.x, col = t % CW, s = t / CW;
   const int i = blockIdx.x * CW + col, j = blockIdx.y;
   for (int k = t; k < nz; k += CW * nseg) {
     sa[k] = a[k];
+    sb[k] = sg.b[k];
     sc[k] = c[k];
   }
   const bool on = i < nx;
   const long nxy = (long)nx * ny;
-  const long P = (long)cxn * cyn;
   const int k0 = s * M;
   const int n = on ? min(M, nz - k0) : 0;
   T* v = var + (long)j * nx + i;
-  const double* rp = rpiv + (long)compact_index(j) * cxn + compact_index(on ? i : 0);
+  const int ci = compact_index(on ? i : 0), cj = compact_index(j);
   // the segment's values (in the field precision: FP32-mixed rounds the
-  // forward-eliminated values to FP32 between the sweeps, as k_trid does) and
-  // the reciprocal pivots of planes k0-1 .. k0+n-1
+  // forward-eliminated values to FP32 between the sweeps, as k_trid does),
+  // loaded first; then the reciprocal pivots of planes k0-1 .. k0+n-1
   T x[M];
   double rb[M + 1];
 #pragma unroll
   for (int m = 0; m < M; ++m)
     if (m < n) x[m] = v[(long)(k0 + m) * nxy];
+  const double lxy = sg.elx[ci] + sg.ely[cj];
+  double piv = k0 > 0 && n > 0 ? sg.pstart[(long)s * cxn * cyn + (long)cj * cxn + ci] : 0.0;
+  rb[0] = piv != 0.0 ? 1.0 / piv : 0.0;
+  __syncthreads();  // sa, sb, sc
+  // init_poisson_solver's recurrence (src/mod_poisson_solver.cc:273-282)
 #pragma unroll
-  for (int m = 0; m <= M; ++m)
-    if (m <= n && k0 - 1 + m >= 0) rb[m] = rp[(long)(k0 - 1 + m) * P];
-  __syncthreads();  // sa, sc
+  for (int m = 1; m <= M; ++m)
+    if (m <= n) {
+      const int k = k0 + m - 1;
+      double q = sb[k] + lxy;
+      if (k > 0) {
+        const double a_tmp = sa[k] / piv;
+        q -= (a_tmp * sc[k - 1]);
+      }
+      if (sg.pin && k == nz - 1 && ci == 0 && cj == 0) q = 0.0;
+      rb[m] = q != 0.0 ? 1.0 / q : 0.0;
+      piv = q;
+    }
   // A: forward elimination from a zero incoming value
   {
     double y = 0.0, g = 1.0;
@@ -334,15 +369,22 @@ int trid_setup(pll_context* c) {
   // FP32 round-off of the mean right-hand side into a constant of ~1e6 in p,
   // which an FP32 p cannot carry next to O(1) pressure differences
   if (c->cfg.pin_singular || c->prec != PLL_FP64) piv[(size_t)(nz - 1) * P] = 0.0;
+  // the partitioned solve of the step: b, the compact eigenvalues and the
+  // pivots of the planes below its segments (it rebuilds the others)
+  std::vector<double> seg;
+  if (nz <= kSegPlanes) {
+    const int M = seg_planes(nz), nseg = (nz + M - 1) / M;
+    seg.assign(nz + c->cxn + c->cyn + (size_t)nseg * P, 0.0);
+    for (int k = 0; k < nz; ++k) seg[k] = b[k];
+    for (int cx = 0; cx < c->cxn; ++cx) seg[nz + cx] = lx[rep(cx)];
+    for (int cy = 0; cy < c->cyn; ++cy) seg[nz + c->cxn + cy] = ly[rep(cy)];
+    double* ps = seg.data() + nz + c->cxn + c->cyn;
+    for (int s = 1; s < nseg; ++s)
+      for (long q = 0; q < P; ++q) ps[s * P + q] = piv[(size_t)(s * M - 1) * P + q];
+    c->seg_pin = (c->cfg.pin_singular || c->prec != PLL_FP64) ? 1 : 0;
+  }
   // FP32-mixed multiplies by reciprocal pivots (0 stands for a zero pivot:
   // the reference's b == 0 branch gives 0, src/mod_poisson_solver.cc:341-346)
-  // The partitioned solve of the step multiplies by reciprocal pivots in
-  // both precisions; FP64 keeps the pivots as well for the entry point.
-  std::vector<double> rpiv;
-  if (c->prec == PLL_FP64 && nz <= kSegPlanes) {
-    rpiv.resize(piv.size());
-    for (size_t t = 0; t < piv.size(); ++t) rpiv[t] = piv[t] != 0.0 ? 1.0 / piv[t] : 0.0;
-  }
   if (c->prec != PLL_FP64)
     for (double& x : piv) x = x != 0.0 ? 1.0 / x : 0.0;
 
@@ -352,10 +394,9 @@ int trid_setup(pll_context* c) {
   PLL_HIP_TRY(hipMemcpy(c->d_a, a.data(), nz * sizeof(double), hipMemcpyHostToDevice));
   PLL_HIP_TRY(hipMemcpy(c->d_c, cc.data(), nz * sizeof(double), hipMemcpyHostToDevice));
   PLL_HIP_TRY(hipMemcpy(c->d_piv, piv.data(), piv.size() * sizeof(double), hipMemcpyHostToDevice));
-  if (!rpiv.empty()) {
-    PLL_HIP_TRY(ctx_malloc(c, &c->d_rpiv, rpiv.size() * sizeof(double)));
-    PLL_HIP_TRY(hipMemcpy(c->d_rpiv, rpiv.data(), rpiv.size() * sizeof(double),
-                          hipMemcpyHostToDevice));
+  if (!seg.empty()) {
+    PLL_HIP_TRY(ctx_malloc(c, &c->d_seg, seg.size() * sizeof(double)));
+    PLL_HIP_TRY(hipMemcpy(c->d_seg, seg.data(), seg.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   return PLL_OK;
 }
@@ -382,39 +423,36 @@ template <typename T>
 constexpr int seg_cols() {
   return std::is_same<T, float>::value ? 32 : 16;
 }
-constexpr int kSegM = 24, kSegS = 16;
-static_assert(kSegM * kSegS >= kSegPlanes, "the segments hold the deepest column");
-
 template <typename T, int M>
-void trid_seg(pll_context* c, const double* rpiv) {
+void trid_seg(pll_context* c) {
   constexpr int kCw = seg_cols<T>();
   const int nseg = (c->g.nz + M - 1) / M;
   const dim3 grid((c->g.nx + kCw - 1) / kCw, c->g.ny);
+  const int nz = c->g.nz;
+  const SegCoef sg{c->d_seg, c->d_seg + nz, c->d_seg + nz + c->cxn,
+                   c->d_seg + nz + c->cxn + c->cyn, c->seg_pin};
   hipLaunchKernelGGL((k_trid_seg<T, M, kCw, kSegS>), grid, dim3(kCw * nseg), 0, c->stream, c->g.nx,
-                     c->g.ny, c->g.nz, nseg, static_cast<T*>(c->spec), c->d_a, c->d_c, rpiv,
-                     c->cxn, c->cyn);
+                     c->g.ny, nz, nseg, static_cast<T*>(c->spec), c->d_a, c->d_c, sg, c->cxn,
+                     c->cyn);
 }
 template <typename T>
-void trid_seg_t(pll_context* c, const double* rpiv) {
-  if (c->g.nz <= kSegS * (kSegM / 3))
-    trid_seg<T, kSegM / 3>(c, rpiv);
-  else if (c->g.nz <= kSegS * (2 * kSegM / 3))
-    trid_seg<T, 2 * kSegM / 3>(c, rpiv);
-  else
-    trid_seg<T, kSegM>(c, rpiv);
+void trid_seg_t(pll_context* c) {
+  switch (seg_planes(c->g.nz)) {
+    case kSegM / 3: trid_seg<T, kSegM / 3>(c); break;
+    case 2 * kSegM / 3: trid_seg<T, 2 * kSegM / 3>(c); break;
+    default: trid_seg<T, kSegM>(c); break;
+  }
 }
 }  // namespace
 
 int launch_trid_step(pll_context* c) {
-  // FP32-mixed keeps reciprocal pivots in d_piv; FP64 has them in d_rpiv when
-  // the columns fit the segments (nz <= 384), else the sequential kernel runs
-  const double* rpiv = c->prec == PLL_FP64 ? c->d_rpiv : c->d_piv;
-  if (!rpiv || c->g.nz > kSegPlanes) return launch_trid(c);
+  // columns deeper than the segments hold (nz > 384) take the sequential kernel
+  if (!c->d_seg || c->g.nz > kSegPlanes) return launch_trid(c);
   TimerScope ts(c, K_TRID);
   if (c->prec == PLL_FP64)
-    trid_seg_t<double>(c, rpiv);
+    trid_seg_t<double>(c);
   else
-    trid_seg_t<float>(c, rpiv);
+    trid_seg_t<float>(c);
   PLL_HIP_TRY(hipGetLastError());
   return PLL_OK;
 }

@@ -145,7 +145,7 @@ void release(pll_context* c) {
     if (p) (void)hipFree(p);
   for (double*& p : c->d_stat)
     if (p) (void)hipFree(p);
-  void* bufs[] = {c->spec,   c->d_mesh, c->d_mesh64, c->d_a,    c->d_c,       c->d_piv, c->d_rpiv,
+  void* bufs[] = {c->spec,   c->d_mesh, c->d_mesh64, c->d_a,    c->d_c,       c->d_piv, c->d_seg,
                   c->d_tw_x, c->d_tw_y,    c->d_rtw_x, c->d_twl_x, c->d_twl_y, c->d_rkcoef,
                   c->d_partial, c->d_scalar, c->d_red, c->d_feat, c->d_nut};
   for (void* p : bufs)

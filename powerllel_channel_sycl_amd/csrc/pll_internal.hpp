@@ -116,7 +116,10 @@ struct pll_context {
   double* d_a = nullptr;
   double* d_c = nullptr;
   double* d_piv = nullptr;  // FP64 pivots (FP32-mixed: their reciprocals), cxn*cyn*nz
-  double* d_rpiv = nullptr;  // FP64: the reciprocal pivots (partitioned solve of the step)
+  // the partitioned solve of the step: b (nz), compact eigenvalues (cxn, cyn)
+  // and the pivots of the planes below its segments (k_trid.hip)
+  double* d_seg = nullptr;
+  int seg_pin = 0;  // the (0,0) column's last pivot is pinned to 0
 
   // FFT plans and twiddles (in field precision)
   pll::FftPlan plan_x;  // complex length nx/2
