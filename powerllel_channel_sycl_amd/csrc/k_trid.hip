@@ -150,13 +150,18 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
 // The segment boundaries are combined with other roundings than the
 // sequential sweep, and the pivots are applied as reciprocals, so the result
 // differs from k_trid's by FP64 round-off (it is not bit-identical).
-constexpr int kSegPlanes = 384;  // the deepest column the segments hold
-// planes per segment (at most kSegM) and segments per column (at most kSegS)
-constexpr int kSegM = 24, kSegS = 16;
-static_assert(kSegM * kSegS >= kSegPlanes, "the segments hold the deepest column");
+constexpr int kSegPlanes = 768;  // the deepest column the segments hold
+// Planes per segment M (8, 16 or 24) and segments per column (at most S = 16,
+// or 32 for columns deeper than 384 planes): short segments where they suffice
+// keep the registers of a thread (its M values and M+1 reciprocal pivots) low.
 constexpr int seg_planes(int nz) {
-  return nz <= kSegS * (kSegM / 3) ? kSegM / 3 : (nz <= kSegS * (2 * kSegM / 3) ? 2 * kSegM / 3 : kSegM);
+  return nz <= 128 ? 8 : (nz <= 256 ? 16 : (nz <= 384 ? 24 : (nz <= 512 ? 16 : 24)));
 }
+constexpr int seg_count_max(int nz) { return nz <= 384 ? 16 : 32; }
+static_assert(seg_planes(kSegPlanes) * seg_count_max(kSegPlanes) >= kSegPlanes &&
+                  seg_planes(512) * seg_count_max(512) >= 512 &&
+                  seg_planes(384) * seg_count_max(384) >= 384,
+              "the segments hold every column they take");
 
 // The sequential combination of the S segment-end values, Y_q = hat_q + dep_q *
 // Y_{q-1} (forward, REV false) or from the top (REV true), by the threads of
@@ -423,7 +428,7 @@ template <typename T>
 constexpr int seg_cols() {
   return std::is_same<T, float>::value ? 32 : 16;
 }
-template <typename T, int M>
+template <typename T, int M, int S>
 void trid_seg(pll_context* c) {
   constexpr int kCw = seg_cols<T>();
   const int nseg = (c->g.nz + M - 1) / M;
@@ -431,23 +436,33 @@ void trid_seg(pll_context* c) {
   const int nz = c->g.nz;
   const SegCoef sg{c->d_seg, c->d_seg + nz, c->d_seg + nz + c->cxn,
                    c->d_seg + nz + c->cxn + c->cyn, c->seg_pin};
-  hipLaunchKernelGGL((k_trid_seg<T, M, kCw, kSegS>), grid, dim3(kCw * nseg), 0, c->stream, c->g.nx,
+  hipLaunchKernelGGL((k_trid_seg<T, M, kCw, S>), grid, dim3(kCw * nseg), 0, c->stream, c->g.nx,
                      c->g.ny, nz, nseg, static_cast<T*>(c->spec), c->d_a, c->d_c, sg, c->cxn,
                      c->cyn);
 }
 template <typename T>
 void trid_seg_t(pll_context* c) {
-  switch (seg_planes(c->g.nz)) {
-    case kSegM / 3: trid_seg<T, kSegM / 3>(c); break;
-    case 2 * kSegM / 3: trid_seg<T, 2 * kSegM / 3>(c); break;
-    default: trid_seg<T, kSegM>(c); break;
+  const int nz = c->g.nz;
+  if (nz <= 384) {
+    switch (seg_planes(nz)) {
+      case 8: trid_seg<T, 8, 16>(c); break;
+      case 16: trid_seg<T, 16, 16>(c); break;
+      default: trid_seg<T, 24, 16>(c); break;
+    }
+  } else if (nz <= 512) {
+    trid_seg<T, 16, 32>(c);
+  } else {
+    trid_seg<T, 24, 32>(c);
   }
 }
 }  // namespace
 
 int launch_trid_step(pll_context* c) {
-  // columns deeper than the segments hold (nz > 384) take the sequential kernel
-  if (!c->d_seg || c->g.nz > kSegPlanes) return launch_trid(c);
+  // columns deeper than the segments hold take the sequential kernel, and so
+  // do FP32-mixed columns deeper than 384 planes (its 1024-thread blocks of 32
+  // segments measured slower than the sequential sweeps at 1536x768x512)
+  const int deepest = c->prec == PLL_FP64 ? kSegPlanes : 384;
+  if (!c->d_seg || c->g.nz > deepest) return launch_trid(c);
   TimerScope ts(c, K_TRID);
   if (c->prec == PLL_FP64)
     trid_seg_t<double>(c);

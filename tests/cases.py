@@ -23,6 +23,10 @@ CASES = {
     "c4slab": (1536, 768, 4, 2 * math.pi, math.pi, 2.0, 2.0, 20000.0, 2.5e-4),
     # a tall column (nz = 300)
     "tall": (16, 12, 300, 2 * math.pi, math.pi, 2.0, 1.8, 2800.0, 2e-3),
+    # columns deeper than 384 planes: the step's partitioned solve with 32
+    # segments of 16 (nz = 512) and of 24 planes (nz = 700)
+    "deep512": (16, 12, 512, 2 * math.pi, math.pi, 2.0, 1.8, 2800.0, 2e-3),
+    "deep700": (16, 12, 700, 2 * math.pi, math.pi, 2.0, 1.8, 2800.0, 2e-3),
     # the Re_tau~550 grid of the benchmark, whole
     "c3": (768, 384, 384, 2 * math.pi, math.pi, 2.0, 1.8, 10000.0, 5e-4),
     # the Re_tau~1000 grid, whole (604 M points)

@@ -218,7 +218,7 @@ def test_init_flow_matches_oracle(ctxs, name):
 
 
 @pytest.mark.parametrize("name,steps", [("c1", 10), ("r235", 10), ("r3", 5), ("gen", 10), ("c2", 10),
-                                        ("ragtile", 5)])
+                                        ("ragtile", 5), ("tall", 5), ("deep512", 5), ("deep700", 5)])
 def test_full_step_parity(name, steps):
     """10 RK2 steps from the same seeded state: u, v, w and p - <p> to 1e-11."""
     _print_case("full step", name, steps)
