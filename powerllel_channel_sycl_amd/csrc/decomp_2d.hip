@@ -38,30 +38,20 @@ struct pll_decomp {
 namespace pll {
 namespace {
 
-// distribute (src/decomp_2d.cc:257-279): the last blocks get the remainder.
+// Block split of 1..num over nprocs ranks, the same split as src/decomp_2d.cc:257-279:
+// every rank holds floor(num/nprocs) indices and the last num % nprocs ranks one more.
+// Closed form per rank r, with `first_long` = the first rank that holds the extra index:
+//   count(r) = base + [r >= first_long],  start(r) = 1 + r*base + max(0, r - first_long).
 void distribute(int num, int nprocs, std::vector<int>& st, std::vector<int>& sz) {
-  st.assign(nprocs, 0);
-  sz.assign(nprocs, 0);
-  std::vector<int> en(nprocs, 0);
-  int size = num / nprocs;
-  const int nu = num - size * nprocs;
-  const int nl = nprocs - nu;
-  st[0] = 1;
-  sz[0] = size;
-  en[0] = size;
-  for (int i = 1; i < nl; i++) {
-    st[i] = st[i - 1] + size;
-    sz[i] = size;
-    en[i] = en[i - 1] + size;
+  const int base = num / nprocs;
+  const int first_long = nprocs - num % nprocs;
+  st.resize(nprocs);
+  sz.resize(nprocs);
+  for (int r = 0; r < nprocs; ++r) {
+    const int extra_before = r > first_long ? r - first_long : 0;
+    sz[r] = base + (r >= first_long ? 1 : 0);
+    st[r] = 1 + r * base + extra_before;
   }
-  size++;
-  for (int i = nl; i < nprocs; i++) {
-    st[i] = en[i - 1] + 1;
-    sz[i] = size;
-    en[i] = en[i - 1] + size;
-  }
-  en[nprocs - 1] = num;
-  sz[nprocs - 1] = num - st[nprocs - 1] + 1;
 }
 
 // The last index m < n with t[m] <= key of a non-decreasing table (t[0] <= key):

@@ -92,6 +92,23 @@ def test_library_bookkeeping_matches_oracle(oracle_lib, grid):
     d.close()
 
 
+def test_block_split_sweep(oracle_lib):
+    # Every split of n over p ranks (the x extent over p_row in a y-pencil): counts are
+    # floor(n/p) with the last n % p ranks one longer, starts are contiguous from 1,
+    # and library and oracle agree rank by rank.
+    for n in range(1, 41):
+        for p in range(1, min(n, 9) + 1):
+            d = pll.Decomp(n, n, n, p, 1)
+            o = oracle_lib.OracleDecomp(n, n, n, p, 1)
+            starts = [d.pencil(1, r)[0][0] for r in range(p)]
+            counts = [d.pencil(1, r)[1][0] for r in range(p)]
+            want = [n // p + (1 if r >= p - n % p else 0) for r in range(p)]
+            assert counts == want, (n, p)
+            assert starts == [1 + sum(want[:r]) for r in range(p)], (n, p)
+            assert all(d.pencil(1, r) == o.pencil(1, r) for r in range(p)), (n, p)
+            d.close()
+
+
 def test_invalid_processor_grid():
     # src/decomp_2d.cc:117-124: min(nx,ny) >= p_row and min(ny,nz) >= p_col
     with pytest.raises(pll.PllError, match="Invalid 2D processor grid"):
