@@ -440,6 +440,215 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
   }
 }
 
+// The FP64 stages of k_rk (the same arithmetic in the same order, so the
+// results are bit-identical) with another schedule of the plane loads.  A
+// fetch only issues the loads of plane k+2 into registers; the projection,
+// forcing and boundary rules apply when that plane is committed to the ring,
+// after plane k has been computed, and plane k's new values are stored after
+// the commit.  In k_rk the arithmetic on the loaded values inside the fetch
+// makes every wave wait for its loads there, and a commit after the stores
+// would wait for their completion too: this form is 4-5 % faster at
+// 768x384x384 (4 waves per SIMD).  Every plane load is unconditional (a
+// thread without a tile cell loads the corner cell (0, 0) and writes LDS slot
+// PL; w is loaded on every plane and replaced by the wall value where it
+// applies): loads that some paths skip leave the compiler's wait counting
+// unsure which are still in flight, and it then waits for them before the next
+// fetch reuses their registers.  The FP64 tendency carries the lower z face,
+// so plane k-1 is not read while plane k is computed and one barrier per plane
+// suffices.  Stage 2 issues the point values of plane k before the fetch, so
+// that waiting for them leaves the plane in flight.
+template <int STAGE, bool PROJ>
+__global__ __launch_bounds__(kRkTx * kRkTy) void k_rk_late(
+    Geom g, const double* __restrict__ su, const double* __restrict__ sv,
+    const double* __restrict__ sw, double* __restrict__ du, double* __restrict__ dv,
+    double* __restrict__ dw, MeshT<double> m, double dxi, double dyi, double rei, double dt,
+    double ucrf, int kchunk, int ntx, int nty, int fold, int defer, double bc_u,
+    const double* __restrict__ force, int forced_mask, const RkCoef* __restrict__,
+    RkUniform, int crf, const double* __restrict__ pp, int corr, double dtdxi, double dtdyi,
+    int, const double* __restrict__, double* __restrict__, int) {
+  using T = double;
+  constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY,
+                NT = TX * TY;
+  constexpr int NPF = (PL + NT - 1) / NT;
+  // stage 1 launches the PROJ variant exactly when the projection is deferred
+  constexpr bool PJ = STAGE == 1 && PROJ;
+  corr = PROJ ? corr : 0;
+  const int bcload = fold;
+  __shared__ T sh[3][3][PL + 1];  // [field][ring slot][tile cell]; slot PL: no cell
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
+  const int kc = (int)(b / (unsigned)(ntx * nty));
+  const int i0 = ti * TX, j0 = tj * TY;  // tile cell (0,0) is the halo corner (i0, j0)
+  const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int i = i0 + tx + 1, j = j0 + ty + 1;
+  const bool active = i <= g.nx && j <= g.ny;
+  const bool any = fold || defer;
+  const bool fu = any && (forced_mask & 1), fv = any && (forced_mask & 2),
+             fw = any && (forced_mask & 4);
+  const T f_u = fu ? force[0] : T(0), f_v = fv ? force[1] : T(0), f_w = fw ? force[2] : T(0);
+
+  long gaddr[NPF];  // plane-relative source address of this thread's tile cells
+  int cell[NPF];
+#pragma unroll
+  for (int n = 0; n < NPF; ++n) {
+    const int e = threadIdx.x + n * NT;
+    int gx = i0 + e % SX, gy = j0 + e / SX;
+    const bool inside = e < PL && gx <= g.nx + 1 && gy <= g.ny + 1;
+    if (bcload) {  // halo cells read their periodic image
+      gx = gx == 0 ? g.nx : (gx == g.nx + 1 ? 1 : gx);
+      gy = gy == 0 ? g.ny : (gy == g.ny + 1 ? 1 : gy);
+    }
+    gaddr[n] = inside ? gx + g.sx * gy : 0;
+    cell[n] = inside ? e : PL;
+  }
+  T ru[NPF], rv[NPF], rw[NPF];
+  T rp0[NPF], rpe[NPF], rpn[NPF], rpt[NPF];  // p at the cell, +x, +y, +z (PJ)
+  auto plane_src = [&](int kk) {
+    const bool ghost = bcload && (kk == 0 || kk == g.nz + 1);
+    return ghost ? (kk == 0 ? 1 : g.nz) : kk;
+  };
+  auto fetch = [&](int kk) {
+    const long pofs = g.sxy * plane_src(kk);
+#pragma unroll
+    for (int n = 0; n < NPF; ++n) {
+      const long a = gaddr[n] + pofs;
+      ru[n] = su[a];
+      rv[n] = sv[a];
+      rw[n] = sw[a];
+      if constexpr (PJ) {
+        rp0[n] = pp[a];
+        rpe[n] = pp[a + 1];
+        rpn[n] = pp[a + g.sx];
+        rpt[n] = pp[a + g.sxy];
+      }
+    }
+  };
+  // k_rk's fetch arithmetic, applied to the loaded registers of plane kk
+  auto commit = [&](int kk, int slot) {
+    const bool ghost = bcload && (kk == 0 || kk == g.nz + 1);
+    const int ks = plane_src(kk);
+    const bool wzero = bcload && (kk == 0 || kk >= g.nz);
+    const T dzci = PJ ? m.dzc_inv[ks] : T(0);
+#pragma unroll
+    for (int n = 0; n < NPF; ++n) {
+      T qu = ru[n], qv = rv[n], qw = rw[n];
+      if constexpr (PJ) {
+        const T p0 = rp0[n];
+        qu = qu - (rpe[n] - p0) * dtdxi;
+        qv = qv - (rpn[n] - p0) * dtdyi;
+        // same association as k_correct: ((p(k+1) - p(k)) * dt) * dzc_inv(k)
+        qw = qw - (rpt[n] - p0) * dt * dzci;
+      }
+      if (fold) {
+        if (fu) qu = qu + f_u;
+        if (fv) qv = qv + f_v;
+        if (fw) qw = qw + f_w;
+      }
+      if (wzero) qw = T(0);
+      if (ghost) {
+        qu = T(2) * bc_u - qu;  // src/mod_updateBound.cc:92,123
+        qv = T(2) * T(0) - qv;
+      }
+      sh[0][slot][cell[n]] = qu;
+      sh[1][slot][cell[n]] = qv;
+      sh[2][slot][cell[n]] = qw;
+    }
+  };
+  if (k0 > k1) return;
+  fetch(k0 - 1);
+  commit(k0 - 1, 0);
+  fetch(k0);
+  commit(k0, 1);
+  fetch(k0 + 1);
+  commit(k0 + 1, 2);
+  __syncthreads();
+  int sm = 0, s0 = 1, sp = 2;
+  const int c = (ty + 1) * SX + tx + 1;
+  // stage 2's point values and, with corr, the pressures around the point,
+  // loaded for every thread (a thread without a point reads cell (1, 1))
+  T qu0 = T(0), qv0 = T(0), qw0 = T(0), qp0 = T(0), qpe = T(0), qpn = T(0), qpt = T(0);
+  auto load_point = [&](int kk) {
+    const long a = active ? g.idx(i, j, kk) : g.idx(1, 1, kk);
+    qu0 = du[a];
+    qv0 = dv[a];
+    qw0 = dw[a];
+    if (corr) {
+      qp0 = pp[a];
+      qpe = pp[a + 1];
+      qpn = pp[a + g.sx];
+      qpt = pp[a + g.sxy];
+    }
+  };
+  // the lower z-face terms of the thread's cell, carried up the column
+  ZFace<T> zf{};
+  if (active) {
+    const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
+    const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
+    const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
+    zf = zface_lower<T>(U, V, W, m, k0);
+  }
+  __syncthreads();  // plane k0-1 is read above and rewritten in the loop
+  T nu = T(0), nv = T(0), nw = T(0);  // plane k's new values until they are stored
+  auto compute = [&](int k) {
+    if (!active) return;
+    const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
+    const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
+    const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
+    // reference arithmetic, src/mod_calcVel.cc:81,103,125 and :194,216,238
+    T tu, tv, tw;
+    if (crf)
+      tendency_z<T, true, true>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
+    else
+      tendency_z<T, false, true>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
+    if (STAGE == 1) {
+      nu = U(0, 0, 0) + dt * tu;
+      nv = V(0, 0, 0) + dt * tv;
+      nw = W(0, 0, 0) + dt * tw;
+    } else {
+      T ou = qu0, ov = qv0, ow = qw0;
+      if (corr) {
+        ou = ou - (qpe - qp0) * dtdxi;
+        ov = ov - (qpn - qp0) * dtdyi;
+        ow = ow - (qpt - qp0) * dt * m.dzc_inv[k];
+      }
+      if (defer) {
+        if (fu) ou = ou + f_u;
+        if (fv) ov = ov + f_v;
+        if (fw) ow = ow + f_w;
+        if (k == g.nz) ow = T(0);  // the wall value of the boundary update
+      }
+      nu = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
+      nv = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
+      nw = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
+    }
+  };
+  auto store = [&](int k) {
+    if (active) {
+      const long a = g.idx(i, j, k);
+      du[a] = nu;
+      dv[a] = nv;
+      dw[a] = nw;
+    }
+  };
+  // the last plane, whose k+2 lies beyond the chunk's ring, runs after the loop
+  for (int k = k0; k < k1; ++k) {
+    if (STAGE == 2) load_point(k);
+    fetch(k + 2);
+    compute(k);
+    commit(k + 2, sm);
+    store(k);
+    __syncthreads();
+    const int t = sm;
+    sm = s0;
+    s0 = sp;
+    sp = t;
+  }
+  if (STAGE == 2) load_point(k1);
+  compute(k1);
+  store(k1);
+}
+
 // Both RK stages of an FP32-mixed step inside a run (src/mod_calcVel.cc:61-125
 // then :174-238) in one pass.  The stored velocity is the previous step's
 // predictor u*; its projection, forcing and boundary update are applied on
@@ -747,32 +956,39 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
   // u1, v1, w1 of plane kk with the boundary rules applied; p of plane kk as
   // stored (updateBoundP ran; x/y halo cells read their periodic image, which
   // holds the same value)
+  // the fetch only issues the loads (w on every plane, from plane 1 or nz for
+  // the ghost planes); the boundary rules apply at the commit, after the plane
+  // below has been computed: arithmetic on the loaded values inside the fetch
+  // would make every wave wait for its loads there
   auto fetch = [&](int kk) {
     const bool ghost = kk == 0 || kk == g.nz + 1;
     const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
-    const bool wzero = kk == 0 || kk >= g.nz;
 #pragma unroll
     for (int n = 0; n < NPF; ++n)
       if (gaddr[n] >= 0) {
         const long a = gaddr[n] + g.sxy * ks;
-        T qu = su[a], qv = sv[a];
-        const T qw = wzero ? T(0) : sw[a];
+        pre[0][n] = su[a];
+        pre[1][n] = sv[a];
+        pre[2][n] = sw[a];
+        if constexpr (PROJ) pre[NF - 1][n] = pin[gaddr[n] + g.sxy * kk];
+      }
+  };
+  auto commit = [&](int kk, int slot) {
+    const bool ghost = kk == 0 || kk == g.nz + 1;
+    const bool wzero = kk == 0 || kk >= g.nz;
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        T qu = pre[0][n], qv = pre[1][n], qw = pre[2][n];
+        if (wzero) qw = T(0);
         if (ghost) {
           qu = T(2) * bc_u - qu;  // src/mod_updateBound.cc:92,123
           qv = T(2) * T(0) - qv;
         }
-        pre[0][n] = qu;
-        pre[1][n] = qv;
-        pre[2][n] = qw;
-        if constexpr (PROJ) pre[NF - 1][n] = pin[gaddr[n] + g.sxy * kk];
-      }
-  };
-  auto commit = [&](int slot) {
-#pragma unroll
-    for (int n = 0; n < NPF; ++n)
-      if (gaddr[n] >= 0) {
-#pragma unroll
-        for (int f = 0; f < NF; ++f) sh[f][slot][cell[n]] = pre[f][n];
+        sh[0][slot][cell[n]] = qu;
+        sh[1][slot][cell[n]] = qv;
+        sh[2][slot][cell[n]] = qw;
+        if constexpr (PROJ) sh[NF - 1][slot][cell[n]] = pre[NF - 1][n];
       }
   };
   // the projection's pressure differences at tile cell c of ring slots s0
@@ -836,11 +1052,11 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
   if (k0 > 1) {
     // recompute w_new on plane k0-1 from u1 planes k0-2, k0-1, k0
     fetch(k0 - 2);
-    commit(0);
+    commit(k0 - 2, 0);
     fetch(k0 - 1);
-    commit(1);
+    commit(k0 - 1, 1);
     fetch(k0);
-    commit(2);
+    commit(k0, 2);
     __syncthreads();
     if (active) {
       T nu, nv, nw;
@@ -853,18 +1069,18 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     }
     __syncthreads();
     fetch(k0 + 1);
-    commit(0);  // slot 0 held plane k0-2
+    commit(k0 + 1, 0);  // slot 0 held plane k0-2
     __syncthreads();
     sm = 1;
     s0 = 2;
     sp = 0;
   } else {
     fetch(k0 - 1);
-    commit(0);
+    commit(k0 - 1, 0);
     fetch(k0);
-    commit(1);
+    commit(k0, 1);
     fetch(k0 + 1);
-    commit(2);
+    commit(k0 + 1, 2);
     __syncthreads();
     if (active) {
       const int c = cyi * SX + cxi;
@@ -919,7 +1135,7 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
                              (vn[ty + 1][tx] - vn[ty][tx]) * dtidyi + (nw - wprev) * dti * m.dzf_inv[k];
       wprev = nw;
     }
-    if (more) commit(sm);
+    if (more) commit(k + 2, sm);
     __syncthreads();
     const int t = sm;
     sm = s0;
@@ -1457,24 +1673,37 @@ int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
   const RkCoef* coef = static_cast<const RkCoef*>(c->d_rkcoef) + (stage - 1) * (c->g.nz + 2);
   const RkUniform cu = rk_uniform(c, stage);
   const int crf = c->u_crf != 0.0;
+  // FP64 runs the late-commit form of the stages (k_rk_late), FP32 k_rk
+  constexpr bool kF64 = std::is_same<T, double>::value;
   if (stage == 1) {
-    // FP64 compiles the deferred projection into its own variant of stage 1
-    auto kernel = corr ? k_rk<T, 1, true> : k_rk<T, 1>;
     TimerScope ts(c, K_RK1);
-    hipLaunchKernelGGL(kernel, dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
-                       F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
-                       F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
-                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
-                       bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr, dtdxi,
-                       dtdyi, 0, dzflzi, c->d_partial, fold);
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
+                         F<T>(c, PLL_U), F<T>(c, PLL_V), F<T>(c, PLL_W), F<T>(c, PLL_U1),
+                         F<T>(c, PLL_V1), F<T>(c, PLL_W1), m, (T)c->dx_inv, (T)c->dy_inv,
+                         (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, fold, 0,
+                         bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr,
+                         dtdxi, dtdyi, 0, dzflzi, c->d_partial, fold);
+    };
+    // the deferred projection is compiled into its own variant of stage 1
+    if constexpr (kF64)
+      go(corr ? k_rk_late<1, true> : k_rk_late<1, false>);
+    else
+      go(corr ? k_rk<T, 1, true> : k_rk<T, 1>);
   } else {
     TimerScope ts(c, K_RK2);
-    hipLaunchKernelGGL((k_rk<T, 2>), dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
-                       F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
-                       F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
-                       (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, 0, fold,
-                       bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr, dtdxi,
-                       dtdyi, acc, dzflzi, c->d_partial, bcload);
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(t.blocks()), dim3(kRkTx * ty), 0, c->stream, c->g,
+                         F<T>(c, PLL_U1), F<T>(c, PLL_V1), F<T>(c, PLL_W1), F<T>(c, PLL_U),
+                         F<T>(c, PLL_V), F<T>(c, PLL_W), m, (T)c->dx_inv, (T)c->dy_inv,
+                         (T)c->re_inv, (T)c->dt, (T)c->u_crf, t.kchunk, t.ntx, t.nty, 0, fold,
+                         bc_u, force, forced_mask(c), coef, cu, crf, F<T>(c, PLL_P), corr,
+                         dtdxi, dtdyi, acc, dzflzi, c->d_partial, bcload);
+    };
+    if constexpr (kF64)
+      go(k_rk_late<2, false>);
+    else
+      go(k_rk<T, 2>);
   }
   PLL_HIP_TRY(hipGetLastError());
   if (acc) {
