@@ -704,9 +704,30 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk12(
     const int e = t + n * NT;
     gaddr[n] = e < UPL ? wrap(i0 - 1 + e % UX, g.nx) + (int)g.sx * wrap(j0 - 1 + e / UX, g.ny) : -1;
   }
-  // source plane kk with the rules of k_rk's folded, projected stage-1 loads
-  float pre[3][NPF];
+  // Source plane kk with the rules of k_rk's folded, projected stage-1 loads.
+  // The fetch only issues the loads (w on every plane, from plane 1 or nz for
+  // the ghost planes); the projection, forcing and boundary rules apply at the
+  // commit, after stage 2 of the plane below has been computed, and stage 2's
+  // results are stored after the commit: arithmetic on the loaded values in the
+  // fetch, or stores ahead of the commit, would make a wave wait there.
+  float ru[NPF], rv[NPF], rw[NPF], rp0[NPF], rpe[NPF], rpn[NPF], rpt[NPF];
   auto fetch = [&](int kk) {
+    const bool ghost = kk == 0 || kk == g.nz + 1;
+    const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
+#pragma unroll
+    for (int n = 0; n < NPF; ++n)
+      if (gaddr[n] >= 0) {
+        const long a = gaddr[n] + g.sxy * ks;
+        ru[n] = su[a];
+        rv[n] = sv[a];
+        rw[n] = sw[a];
+        rp0[n] = pp[a];
+        rpe[n] = pp[a + 1];
+        rpn[n] = pp[a + g.sx];
+        rpt[n] = pp[a + g.sxy];
+      }
+  };
+  auto commit = [&](int kk, int slot) {
     const bool ghost = kk == 0 || kk == g.nz + 1;
     const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
     const bool wzero = kk == 0 || kk >= g.nz;
@@ -714,32 +735,22 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk12(
 #pragma unroll
     for (int n = 0; n < NPF; ++n)
       if (gaddr[n] >= 0) {
-        const long a = gaddr[n] + g.sxy * ks;
-        float qu = su[a], qv = sv[a], qw = wzero ? 0.f : sw[a];
-        const float p0 = pp[a];
-        qu = qu - (pp[a + 1] - p0) * dtdxi;
-        qv = qv - (pp[a + g.sx] - p0) * dtdyi;
-        if (!wzero) qw = qw - (pp[a + g.sxy] - p0) * dt * dzci;
+        const float p0 = rp0[n];
+        float qu = ru[n] - (rpe[n] - p0) * dtdxi;
+        float qv = rv[n] - (rpn[n] - p0) * dtdyi;
+        float qw = rw[n] - (rpt[n] - p0) * dt * dzci;
         if (fu) qu = qu + f_u;
         if (fv) qv = qv + f_v;
-        if (fw && !wzero) qw = qw + f_w;
+        if (fw) qw = qw + f_w;
+        if (wzero) qw = 0.f;
         if (ghost) {
           qu = 2.f * bc_u - qu;  // src/mod_updateBound.cc:92,123
           qv = 2.f * 0.f - qv;
         }
-        pre[0][n] = qu;
-        pre[1][n] = qv;
-        pre[2][n] = qw;
-      }
-  };
-  auto commit = [&](int slot) {
-#pragma unroll
-    for (int n = 0; n < NPF; ++n)
-      if (gaddr[n] >= 0) {
         const int e = t + n * NT;
-        ssrc[0][slot][e] = pre[0][n];
-        ssrc[1][slot][e] = pre[1][n];
-        ssrc[2][slot][e] = pre[2][n];
+        ssrc[0][slot][e] = qu;
+        ssrc[1][slot][e] = qv;
+        ssrc[2][slot][e] = qw;
       }
   };
   // this thread's stage-1 cells: A = its own tile point, B = one ring cell (t < NRING)
@@ -763,11 +774,11 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk12(
   int sm = 0, s0 = 1, sp = 2;  // source slots of planes q-1, q, q+1
   const int qs = max(k0 - 1, 1), qe = k1 + 1;  // u1 planes formed: stage 1 up to nz, rules at nz+1
   fetch(qs - 1);
-  commit(0);
+  commit(qs - 1, 0);
   fetch(qs);
-  commit(1);
+  commit(qs, 1);
   fetch(qs + 1);
-  commit(2);
+  commit(qs + 1, 2);
   __syncthreads();
   ZConv za{}, zb{}, z2{};  // carried lower-face products: stage 1 at A and B, stage 2
   // stage 1 of region cell (rx, ry) on plane q, w1 = 0 on the wall plane nz.
@@ -845,6 +856,7 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk12(
     // fetched here rather than before stage 1: its registers are then not
     // live across both stages
     if (more) fetch(q + 2);
+    float nu = 0.f, nv = 0.f, nw = 0.f;
     if (s2 && active) {
       const PlaneAcc<float, RX> U{su1[0][(k + 2) % 3], su1[0][k % 3], su1[0][(k + 1) % 3], ra};
       const PlaneAcc<float, RX> V{su1[1][(k + 2) % 3], su1[1][k % 3], su1[1][(k + 1) % 3], ra};
@@ -856,14 +868,18 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk12(
         tendency_fast<true, true>(U, V, W, cq, cu2, iu, iv, iw, &z2);
       else
         tendency_fast<false, true>(U, V, W, cq, cu2, iu, iv, iw, &z2);
-      const long a = g.idx(i, j, k);
-      const float nu = (U(0, 0, 0) + pu) * 0.5f + iu;
-      du[a] = nu;
-      dv[a] = (V(0, 0, 0) + pv) * 0.5f + iv;
-      dw[a] = (W(0, 0, 0) + pw) * 0.5f + iw;
+      nu = (U(0, 0, 0) + pu) * 0.5f + iu;
+      nv = (V(0, 0, 0) + pv) * 0.5f + iv;
+      nw = (W(0, 0, 0) + pw) * 0.5f + iw;
       if (acc) msum += (double)nu * dzflzi[k];
     }
-    if (more) commit(sm);  // source plane q+2 replaces q-1
+    if (more) commit(q + 2, sm);  // source plane q+2 replaces q-1
+    if (s2 && active) {
+      const long a = g.idx(i, j, k);
+      du[a] = nu;
+      dv[a] = nv;
+      dw[a] = nw;
+    }
     __syncthreads();
     const int tmp = sm;
     sm = s0;
