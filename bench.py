@@ -75,9 +75,9 @@ STEP_BYTES_FP64 = 312
 # profiles/retau550_768x384x384_step_kernels.md holds the runs.  Reported as
 # the roofline entry's `traffic` for that workload only.
 MEASURED_HBM_B_PER_POINT = {
-    "fp64": {"rk1": 62.0, "rk2_rhs": 94.3, "fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 16.2,
+    "fp64": {"rk1": 61.4, "rk2_rhs": 93.9, "fft_x_fwd": 16.0, "fft_y_fwd": 16.0, "solve_trid": 16.2,
              "fft_y_bwd": 16.0, "fft_x_bwd": 16.0, "correct_vel": 59.8},
-    "fp32_mixed": {"rk1": 27.2, "rk2": 39.0, "rk12": 32.1, "fft_x_fwd_rhs": 20.7, "fft_y_fwd": 8.0,
+    "fp32_mixed": {"rk1": 27.2, "rk2": 39.0, "rk12": 32.2, "fft_x_fwd_rhs": 20.7, "fft_y_fwd": 8.0,
                    "solve_trid": 8.2, "fft_y_bwd": 8.0, "fft_x_bwd": 8.0, "correct_vel": 31.6},
 }
 

@@ -457,8 +457,11 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
 // so plane k-1 is not read while plane k is computed and one barrier per plane
 // suffices.  Stage 2 issues the point values of plane k before the fetch, so
 // that waiting for them leaves the plane in flight.
+// tile rows of k_rk_late: 8 (two 512-thread blocks per CU at 126 VGPRs); 4 rows
+// measured 9 % and 16 rows 5 % slower at 768x384x384
+constexpr int kLateTy = 8;
 template <int STAGE, bool PROJ>
-__global__ __launch_bounds__(kRkTx * kRkTy) void k_rk_late(
+__global__ __launch_bounds__(kRkTx * kLateTy) void k_rk_late(
     Geom g, const double* __restrict__ su, const double* __restrict__ sv,
     const double* __restrict__ sw, double* __restrict__ du, double* __restrict__ dv,
     double* __restrict__ dw, MeshT<double> m, double dxi, double dyi, double rei, double dt,
@@ -467,7 +470,7 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk_late(
     RkUniform, int crf, const double* __restrict__ pp, int corr, double dtdxi, double dtdyi,
     int, const double* __restrict__, double* __restrict__, int) {
   using T = double;
-  constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY,
+  constexpr int TX = kRkTx, TY = kLateTy, SX = TX + 2, SY = TY + 2, PL = SX * SY,
                 NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
   // stage 1 launches the PROJ variant exactly when the projection is deferred
@@ -1674,7 +1677,7 @@ int rk_coef_setup(pll_context* c) {
 
 template <typename T>
 int rk_t(pll_context* c, int stage, int fold, int corr, int acc, int bcload) {
-  const int ty = kRkTy;
+  const int ty = std::is_same<T, double>::value ? kLateTy : kRkTy;
   const TileGrid t = tile_grid(c, ty);
   if (acc && (long)t.blocks() > c->npartial)
     return fail(PLL_ERR_STATE, "reduction scratch smaller than the stencil grid");
