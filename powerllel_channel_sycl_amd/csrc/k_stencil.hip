@@ -201,10 +201,11 @@ constexpr int correct_ty() {
 // stage 2 also sums its new u times dzflzi(k) in FP64 into partial[block]: the
 // bulk mean of the next projected velocity, since the projection changes every
 // x-row's sum only by p(nx+1) - p(1) = 0.
-// PROJ compiles the deferred projection in: always in FP32; in FP64 only the
-// stage-1 variant that needs it, so the other FP64 launches keep their
-// register count and occupancy.
-template <typename T, int STAGE, bool PROJ = std::is_same<T, float>::value>
+// k_rk is the FP32-mixed form (tendency_fast, per-plane coefficients);
+// k_rk_late below is the FP64 form in the reference's arithmetic.  PROJ
+// compiles the deferred projection in; stage 1 launches the PROJ variant when
+// the projection is deferred.
+template <typename T, int STAGE, bool PROJ = true>
 __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restrict__ su,
                                                      const T* __restrict__ sv,
                                                      const T* __restrict__ sw, T* __restrict__ du,
@@ -223,14 +224,11 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
   constexpr int TX = kRkTx, TY = kRkTy, SX = TX + 2, SY = TY + 2, PL = SX * SY,
                 NT = TX * TY;
   constexpr int NPF = (PL + NT - 1) / NT;
-  // the predictor's mean of stage 2 serves the FP32-mixed step only (the FP64
-  // step sums it in the fused stage 2 + calcRHS kernel)
-  constexpr bool kF32 = std::is_same<T, float>::value;
+  static_assert(std::is_same<T, float>::value, "the FP64 stages run k_rk_late");
   corr = PROJ ? corr : 0;
-  acc = kF32 ? acc : 0;
   // halo cells from the boundary rules on load: with the folded forcing, or
   // alone (bcload) for a source whose boundary update was not run
-  bcload = (fold || (kF32 && bcload)) ? 1 : 0;
+  bcload = (fold || bcload) ? 1 : 0;
   __shared__ T sh[3][3][PL];  // [field][ring slot][tile cell]
   const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = (int)(b % (unsigned)ntx), tj = (int)((b / (unsigned)ntx) % (unsigned)nty);
@@ -342,17 +340,12 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
     }
   };
   Point qc{};
-  // FP64: the lower z-face terms of the thread's cell, carried up the column
-  ZFace<T> zf{};
-  ZConv zc{};  // FP32: the carried lower-face convective products
+  ZConv zc{};  // the carried lower-face convective products of the thread's column
   if (active) {
     const PlaneAcc<T, SX> U{sh[0][sm], sh[0][s0], sh[0][sp], c};
     const PlaneAcc<T, SX> V{sh[1][sm], sh[1][s0], sh[1][sp], c};
     const PlaneAcc<T, SX> W{sh[2][sm], sh[2][s0], sh[2][sp], c};
-    if constexpr (kF32)
-      zc = zconv_lower(U, V, W, coef[k0]);
-    else
-      zf = zface_lower<T>(U, V, W, m, k0);
+    zc = zconv_lower(U, V, W, coef[k0]);
   }
   for (int k = k0; k <= k1; ++k) {
     const bool more = k + 2 <= k1 + 1;
@@ -380,25 +373,7 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
           if (k == g.nz) ow = T(0);  // the wall value of the boundary update
         }
       }
-      if constexpr (std::is_same<T, double>::value) {
-        // reference arithmetic, src/mod_calcVel.cc:81,103,125 and :194,216,238
-        T tu, tv, tw;
-        if (crf)
-          tendency_z<T, true, true>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
-        else
-          tendency_z<T, false, true>(U, V, W, m, k, dxi, dyi, rei, ucrf, tu, tv, tw, zf);
-        if (STAGE == 1) {
-          du[a] = U(0, 0, 0) + dt * tu;
-          dv[a] = V(0, 0, 0) + dt * tv;
-          dw[a] = W(0, 0, 0) + dt * tw;
-        } else {
-          const T nu = (U(0, 0, 0) + dt * tu + ou) * T(0.5);
-          du[a] = nu;
-          dv[a] = (V(0, 0, 0) + dt * tv + ov) * T(0.5);
-          dw[a] = (W(0, 0, 0) + dt * tw + ow) * T(0.5);
-          if (acc) msum += (double)nu * dzflzi[k];
-        }
-      } else {
+      {
         const RkCoef q = coef[k];
         float iu, iv, iw;
         if (crf)
@@ -440,15 +415,16 @@ __global__ __launch_bounds__(kRkTx * kRkTy) void k_rk(Geom g, const T* __restric
   }
 }
 
-// The FP64 stages of k_rk (the same arithmetic in the same order, so the
-// results are bit-identical) with another schedule of the plane loads.  A
-// fetch only issues the loads of plane k+2 into registers; the projection,
-// forcing and boundary rules apply when that plane is committed to the ring,
-// after plane k has been computed, and plane k's new values are stored after
-// the commit.  In k_rk the arithmetic on the loaded values inside the fetch
-// makes every wave wait for its loads there, and a commit after the stores
-// would wait for their completion too: this form is 4-5 % faster at
-// 768x384x384 (4 waves per SIMD).  Every plane load is unconditional (a
+// The FP64 stages: k_rk's ring, folding and deferred projection, in the
+// reference's arithmetic and operation order (tendency_z, src/mod_calcVel.cc:
+// 81,103,125 and :194,216,238; bit-identical to the reference's stages), with
+// the plane loads scheduled late.  A fetch only issues the loads of plane k+2
+// into registers; the projection, forcing and boundary rules apply when that
+// plane is committed to the ring, after plane k has been computed, and plane
+// k's new values are stored after the commit.  With that arithmetic inside the
+// fetch (k_rk's form) every wave waits for its loads there, and a commit after
+// the stores waits for their completion too: that form ran 4-5 % slower in FP64
+// at 768x384x384 (4 waves per SIMD).  Every plane load is unconditional (a
 // thread without a tile cell loads the corner cell (0, 0) and writes LDS slot
 // PL; w is loaded on every plane and replaced by the wall value where it
 // applies): loads that some paths skip leave the compiler's wait counting
