@@ -105,6 +105,9 @@ def parse():
                     help="run the learned closure's kernels through their own entry points "
                          "(filter, samples, training, both closure stages; no pll_step) on the "
                          "closure workflow's grids, print their HIP-event times and exit")
+    ap.add_argument("--closure-only", action="store_true",
+                    help="run only the closure measurement (the coarse 96x64x64 step with and "
+                         "without the closure, and the training throughput), print it and exit")
     ap.add_argument("--stat-probe", action="store_true",
                     help="time the run-time statistics accumulation (all 11 sums) on the "
                          "workload grid in both precisions and exit")
@@ -399,6 +402,9 @@ def main():
         return
     if args.closure_kernels:
         closure_kernels_probe()
+        return
+    if args.closure_only:
+        print(json.dumps({"closure": closure_bench()}), flush=True)
         return
     if args.stat_probe:
         stat_probe(args.workload)
