@@ -150,20 +150,18 @@ __global__ __launch_bounds__(kTx* kTy) void k_trid(int nx, int ny, int nz, T* __
 // The segment boundaries are combined with other roundings than the
 // sequential sweep, and the pivots are applied as reciprocals, so the result
 // differs from k_trid's by FP64 round-off (it is not bit-identical).
-// FP32-mixed runs the recurrence with one division's work per plane instead of
-// two: the multiplier is a[k] times the reciprocal of the pivot below (already
-// formed), and the reciprocal is the hardware estimate refined by two Newton
-// steps (within an ulp or two of 1/q, FP64); the FP32 work array's own rounding
-// is some 5e8 times larger.  The recurrence is the kernel's longest dependent
-// chain, and this form of it saves 14 % of the FP32-mixed solve at
-// 768x384x384; the FP64 solve keeps the divisions (its pivots stay bit-exact,
-// and the reciprocal saved it nothing).
+// FP32-mixed runs the recurrence with less than one division's work per plane
+// instead of two divisions: the multiplier is a[k] times the reciprocal of the
+// pivot below (already formed), and the reciprocal is the hardware estimate
+// refined by one Newton step (which squares its relative error: far below the
+// 6e-8 rounding of the FP32 work array).  The recurrence is the kernel's
+// longest dependent chain, and this form of it saves 16 % of the FP32-mixed
+// solve at 768x384x384 (375 -> 316 us; a second Newton step costs 7 us); the
+// FP64 solve keeps the divisions (its pivots stay bit-exact, and a reciprocal
+// saved it nothing).
 __device__ __forceinline__ double rcp_newton(double q) {
-  double r = __builtin_amdgcn_rcp(q);
-  double e = __builtin_fma(-q, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-q, r, 1.0);
-  return __builtin_fma(r, e, r);
+  const double r = __builtin_amdgcn_rcp(q);
+  return __builtin_fma(r, __builtin_fma(-q, r, 1.0), r);
 }
 constexpr int kSegPlanes = 768;  // the deepest column the segments hold
 // Planes per segment M (8, 16 or 24) and segments per column (at most S = 16,
