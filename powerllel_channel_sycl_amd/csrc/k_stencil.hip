@@ -948,13 +948,12 @@ __global__ __launch_bounds__(kRkTx* TY) void k_rk2_rhs(
     gaddr[n] = inside ? wrapx(gx) + g.sx * wrapy(gy) : -1;
   }
   T pre[NF][NPF];
-  // u1, v1, w1 of plane kk with the boundary rules applied; p of plane kk as
+  // u1, v1, w1 of plane kk with the boundary rules applied, and p of plane kk as
   // stored (updateBoundP ran; x/y halo cells read their periodic image, which
-  // holds the same value)
-  // the fetch only issues the loads (w on every plane, from plane 1 or nz for
-  // the ghost planes); the boundary rules apply at the commit, after the plane
-  // below has been computed: arithmetic on the loaded values inside the fetch
-  // would make every wave wait for its loads there
+  // holds the same value).  The fetch only issues the loads (w on every plane,
+  // from plane 1 or nz for the ghost planes); the boundary rules apply at the
+  // commit, after the plane below has been computed: arithmetic on the loaded
+  // values inside the fetch would make every wave wait for its loads there.
   auto fetch = [&](int kk) {
     const bool ghost = kk == 0 || kk == g.nz + 1;
     const int ks = ghost ? (kk == 0 ? 1 : g.nz) : kk;
