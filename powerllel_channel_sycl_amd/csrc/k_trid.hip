@@ -438,14 +438,16 @@ int launch_trid(pll_context* c) {
 
 namespace {
 // Columns per block, planes per segment and segments per column of the
-// partitioned solve.
-template <typename T>
+// partitioned solve: FP32-mixed takes 32 columns of up to 16 segments (128-byte
+// rows) and 16 columns of 32 segments (512-thread blocks; 32 columns, 1024
+// threads, ran 9 % slower at 1536x768x512), FP64 16 columns.
+template <typename T, int S>
 constexpr int seg_cols() {
-  return std::is_same<T, float>::value ? 32 : 16;
+  return std::is_same<T, float>::value && S <= 16 ? 32 : 16;
 }
 template <typename T, int M, int S>
 void trid_seg(pll_context* c) {
-  constexpr int kCw = seg_cols<T>();
+  constexpr int kCw = seg_cols<T, S>();
   const int nseg = (c->g.nz + M - 1) / M;
   const dim3 grid((c->g.nx + kCw - 1) / kCw, c->g.ny);
   const int nz = c->g.nz;
@@ -473,11 +475,10 @@ void trid_seg_t(pll_context* c) {
 }  // namespace
 
 int launch_trid_step(pll_context* c) {
-  // columns deeper than the segments hold take the sequential kernel, and so
-  // do FP32-mixed columns deeper than 384 planes (its 1024-thread blocks of 32
-  // segments measured slower than the sequential sweeps at 1536x768x512)
-  const int deepest = c->prec == PLL_FP64 ? kSegPlanes : 384;
-  if (!c->d_seg || c->g.nz > deepest) return launch_trid(c);
+  // columns deeper than the segments hold take the sequential kernel
+  // (FP32-mixed at 1536x768x512: 2397 us partitioned against 2533 us
+  // sequential, since its recurrence runs on reciprocals)
+  if (!c->d_seg || c->g.nz > kSegPlanes) return launch_trid(c);
   TimerScope ts(c, K_TRID);
   if (c->prec == PLL_FP64)
     trid_seg_t<double>(c);

@@ -77,9 +77,11 @@ def test_fp32_trid_matches_oracle(name):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("name,steps", [("c1", 10), ("r3", 10), ("c2", 10), ("ragged", 10)])
+@pytest.mark.parametrize("name,steps", [("c1", 10), ("r3", 10), ("c2", 10), ("ragged", 10),
+                                        ("deep512", 5), ("deep700", 5)])
 def test_fp32_short_horizon_pointwise(name, steps):
-    """10 steps: u, v, w within 1e-5 (relative to max|u|) of the FP64 oracle,
+    """10 steps (5 on the deep columns, whose step solve is partitioned into
+    32 segments): u, v, w within 1e-5 (relative to max|u|) of the FP64 oracle,
     divergence at the FP32 floor, bulk velocity held to 1e-6."""
     print("CASE fp32 short", name, steps, flush=True)
     o, f = _seeded(name)
