@@ -439,8 +439,9 @@ int launch_trid(pll_context* c) {
 namespace {
 // Columns per block, planes per segment and segments per column of the
 // partitioned solve: FP32-mixed takes 32 columns of up to 16 segments (128-byte
-// rows) and 16 columns of 32 segments (512-thread blocks; 32 columns, 1024
-// threads, ran 9 % slower at 1536x768x512), FP64 16 columns.
+// rows; 16 columns ran 374 against 316 us at 768x384x384) and 16 columns of 32
+// segments (512-thread blocks; 32 columns, 1024 threads, ran 9 % slower at
+// 1536x768x512), FP64 16 columns.
 template <typename T, int S>
 constexpr int seg_cols() {
   return std::is_same<T, float>::value && S <= 16 ? 32 : 16;
