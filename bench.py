@@ -118,7 +118,7 @@ def closure_bench(steps=50, warmup=5):
     """The learned closure on the coarse grid of the Re_tau~395 closure
     workflow (96 x 64 x 64, 2pi x pi x 2, stretch 1.5, re = 6875, dt = 0.008,
     FP64): the step with and without a closure attached (its network
-    evaluated inside the RK stage kernels), and Adam steps of the network's
+    evaluated once per cell for each RK stage), and Adam steps of the network's
     MFMA training kernels on one snapshot's worth of samples (393,216)."""
     import torch  # device buffers for the samples; imported before libpll.so
     import numpy as np

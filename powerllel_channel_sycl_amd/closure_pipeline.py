@@ -12,8 +12,8 @@
    backward kernels with Adam, one snapshot per step; the last snapshots are
    held out.
 4. Two coarse runs start from the filtered DNS state: without a model, and
-   with the closure attached (its eddy viscosity evaluated inside the RK
-   stage kernels).  Their mean profile, Reynolds stresses and wall shear are
+   with the closure attached (its eddy viscosity evaluated once per cell for
+   each RK stage).  Their mean profile, Reynolds stresses and wall shear are
    compared with the filtered DNS.
 
 Device memory for the dataset comes from torch (plumbing only); every
